@@ -1,0 +1,157 @@
+/*
+ * pn1.h -- C ABI of libpn1, the MI355X (gfx950) force / energy path of
+ * Limit-sticky9 (N-body star cluster + central black hole with limited
+ * 1PN corrections, Plummer softening, G = 1 units).
+ *
+ * Conventions shared by every entry point:
+ *   - particle 0 is the black hole (BH); n counts it (n = N_stars + 1);
+ *   - x, v, a are AoS [n][3] doubles, m is [n] doubles (the reference's
+ *     layout, Limit-sticky9.c:742-745);
+ *   - c is the speed of light: every 1PN term carries 1/c^2 (the reference
+ *     has none, i.e. c = 1, which stays the default of the legacy shapes);
+ *   - nothing here takes or returns torch or HIP types.
+ *
+ * Two precision modes:
+ *   PN1_FP64   all pair arithmetic in FP64;
+ *   PN1_MIXED  star-star pair arithmetic in FP32 on double-single (hi/lo)
+ *              positions with compensated FP32 sums; the BH-star 1PN terms,
+ *              the RK update and all energy sums stay in FP64.
+ */
+#ifndef PN1_H
+#define PN1_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (context API) -------------------------------------- */
+#define PN1_OK        0
+#define PN1_EINVAL   (-1)   /* bad argument or input state                 */
+#define PN1_ENOMEM   (-2)   /* device or host allocation failed            */
+#define PN1_EDEVICE  (-3)   /* a HIP call failed                           */
+#define PN1_ENODEV   (-4)   /* no usable gfx950 device                     */
+#define PN1_EIO      (-5)   /* file missing, short or malformed            */
+
+#define PN1_FP64   0
+#define PN1_MIXED  1
+
+/* ====================================================================== */
+/* 1. Legacy shapes: drop-in replacements for the reference's functions.  */
+/*    Same prototypes (the reference's [NMAX][3] parameters decay to      */
+/*    double (*)[3]); same outputs; energy()/initial_energy() print the   */
+/*    same text.  They use a library-owned device context created on     */
+/*    first call, the process-global c and mode set below, and on any     */
+/*    error print to stderr and exit(1) as the reference does on failure. */
+/*    Not thread-safe (neither is the reference).                         */
+/* ====================================================================== */
+
+/* Replaces force(), Limit-sticky9.c:21-424.  Overwrites a[0..n); pot is
+ * ignored, as in the reference. */
+void force(double (*x)[3], double (*v)[3], double *m, double eps,
+           double (*a)[3], double *pot, int n);
+
+/* Replaces energy(), Limit-sticky9.c:427-564.  Prints the time, the
+ * energy block and DE = (init_ene - E)/E. */
+void energy(double t, double (*x)[3], double (*v)[3], double *m, int n,
+            double init_ene, double eps);
+
+/* Replaces initial_energy(), Limit-sticky9.c:566-714.  Prints the t = 0
+ * block and stores the total in *init_ene. */
+void initial_energy(double (*x)[3], double (*v)[3], double *m, int n,
+                    double *init_ene, double eps);
+
+/* Speed of light used by the three legacy shapes (default 1.0, which is
+ * the reference).  Returns PN1_EINVAL unless c > 0. */
+int pn1_set_c(double c);
+/* Precision mode of the legacy force() (default PN1_FP64). */
+int pn1_set_mode(int mode);
+
+/* ====================================================================== */
+/* 2. Context API: device-resident state, no host round trip per stage.   */
+/*    Thread-safe across distinct contexts.                               */
+/* ====================================================================== */
+typedef struct pn1_ctx pn1_ctx;
+
+typedef struct {
+    double kin;      /* 1/2 sum m v^2 over all particles incl. the BH      */
+    double pot;      /* softened Newtonian potential energy                 */
+    double pot_pn;   /* BH-star 1PN energy / c^2                            */
+    double pot_pn2;  /* three-body 1PN energy / c^2                         */
+    double total;    /* kin + pot + pot_pn + pot_pn2                        */
+} pn1_energy_t;
+
+/* Creates a context for up to `capacity` particles on HIP device `device`. */
+int  pn1_create(pn1_ctx **out, int capacity, int device);
+void pn1_destroy(pn1_ctx *ctx);
+/* eps > 0 (Plummer softening), c > 0, dt > 0, mode PN1_FP64 | PN1_MIXED. */
+int  pn1_set_params(pn1_ctx *ctx, double eps, double c, double dt, int mode);
+/* Copies a state in (1 <= n <= capacity; m[0] > 0; no star on the BH). */
+int  pn1_upload(pn1_ctx *ctx, const double *x, const double *v,
+                const double *m, int n);
+int  pn1_download(pn1_ctx *ctx, double *x, double *v);
+/* One force evaluation on the resident state; a is AoS [n][3]. */
+int  pn1_force(pn1_ctx *ctx, double *a);
+/* Energy of the resident state. */
+int  pn1_energy(pn1_ctx *ctx, pn1_energy_t *out);
+/* `steps` steps of the reference's three-evaluation stage scheme
+ * (Limit-sticky9.c:841-893), state kept on the device; the step is
+ * captured once into a hipGraph and replayed. */
+int  pn1_step(pn1_ctx *ctx, int steps);
+/* Blocks until the context's stream is idle. */
+int  pn1_sync(pn1_ctx *ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char *pn1_last_error(const pn1_ctx *ctx);
+
+/* Launch shape of the pair kernels: the j range is split into `jsplit`
+ * chunks (0 = automatic), and the mixed kernel compensates its FP32 sums
+ * every `tile` pairs (0 = default).  Used by the tuner. */
+int  pn1_set_tuning(pn1_ctx *ctx, int jsplit, int tile);
+int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile);
+
+/* Timing of the pair kernel: while enabled, pn1_step/pn1_force launch
+ * without a graph and bracket every force pair-kernel launch with HIP
+ * events on the context's stream.  pn1_profile_read returns the number
+ * of pair launches and their summed duration (ms) since enabling. */
+int  pn1_profile_enable(pn1_ctx *ctx, int on);
+int  pn1_profile_read(pn1_ctx *ctx, int *launches, double *total_ms);
+
+/* ====================================================================== */
+/* 3. Host-side helpers (no device needed).                               */
+/* ====================================================================== */
+/* Validates a state as the context API does; writes a reason to msg. */
+int pn1_check_input(const double *x, const double *v, const double *m,
+                    int n, double eps, char *msg, int msglen);
+
+/* Initial conditions (all positions / velocities FP32-exact, BH at the
+ * origin at rest, stars of mass 1/N).  SplitMix64 stream from `seed`.
+ *   uniform sphere of radius R, Gaussian velocities scaled to virial
+ *   ratio Q = K/|W| (W incl. the BH, softened with eps);
+ *   Plummer sphere (scale radius a, total star mass 1), Aarseth-Henon-
+ *   Wielen sampling with the mass fraction cut at 0.999.               */
+int pn1_ic_uniform_sphere(int n_stars, unsigned long long seed, double R,
+                          double Q, double m_bh, double eps,
+                          double *x, double *v, double *m);
+int pn1_ic_plummer(int n_stars, unsigned long long seed, double a,
+                   double m_bh, double *x, double *v, double *m);
+
+/* .dat input of Limit-sticky9.c:795-809: n-1 lines "x y z", then n-1
+ * lines "vx vy vz", read as float; the BH row is not in the file and is
+ * set to zero.  Masses are not read. */
+int pn1_read_dat(const char *path, int n, double *x, double *v);
+/* Snapshot of Limit-sticky9.c:820-826: n lines "%lf %lf %lf \n" of
+ * positions then n of velocities (BH first).  append != 0 appends. */
+int pn1_write_snapshot(const char *path, int append, const double *x,
+                       const double *v, int n);
+/* Exact binary checkpoint (header + FP64 x, v, m) for resume. */
+int pn1_write_checkpoint(const char *path, int n, double t, long long step,
+                         double eps, double c, double dt, const double *x,
+                         const double *v, const double *m);
+/* Reads a checkpoint; pass x = v = m = NULL to read only the header. */
+int pn1_read_checkpoint(const char *path, int *n, double *t,
+                        long long *step, double *eps, double *c, double *dt,
+                        double *x, double *v, double *m, int capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PN1_H */

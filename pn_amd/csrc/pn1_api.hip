@@ -1,0 +1,606 @@
+// pn1_api.hip -- host side of libpn1: the context API and the three
+// legacy shapes of Limit-sticky9 (force, energy, initial_energy).
+//
+// Device layout (FP64 SoA, stride cap, particle 0 = BH):
+//   X[6][cap]   state at the start of a step (x, y, z, vx, vy, vz)
+//   Y[6][cap]   stage input (the reference's x1, v1)
+//   H1, H2      stage states 1 and 2 (the reference's hx[1..2], hv[1..2])
+//   m[cap], acc[3][cap]
+//   j64[cap] / j32[cap]   per-evaluation j-records (AoS)
+//   part        pair partials [S][12][cap] doubles, or [S][12][2][cap] floats
+// Nothing is allocated after pn1_upload / pn1_set_tuning: pn1_force,
+// pn1_energy and pn1_step only launch kernels and copy results.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "pn1.h"
+#include "pn1_kernels.hpp"
+
+using namespace pn1dev;
+
+struct pn1_ctx {
+    int dev = -1;
+    hipStream_t st = nullptr;
+    int cap = 0, n = 0;
+    bool have_state = false;
+    double eps = 1e-3, c = 1.0, dt = 1.0 / 1024.0;
+    int mode = PN1_FP64;
+    double m0 = 0.0;            // host copy of m[0]
+    int jsplit_req = 0, tile_req = 0;
+    // launch plan for the current n
+    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 32;
+    // device buffers
+    double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr;
+    double *m = nullptr, *acc = nullptr, *gblk = nullptr, *gs = nullptr;
+    double *bhblk = nullptr, *eblk = nullptr, *eout = nullptr;
+    JRec64* j64 = nullptr;
+    JRec32* j32 = nullptr;
+    void* part = nullptr;
+    size_t part_bytes = 0;
+    double* h_stage = nullptr;  // pinned, 7 * cap doubles
+    hipGraphExec_t gexec = nullptr;
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    int prof_launches = 0;
+    double prof_ms = 0.0;
+    char err[512] = {0};
+};
+
+namespace {
+
+int seterr(pn1_ctx* c, int code, const char* fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return seterr((ctx), PN1_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(pn1_ctx* c, T** p, size_t count)
+{
+    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
+    }
+    return PN1_OK;
+}
+
+int nblocks(int nst) { return nst > 0 ? (nst + WG - 1) / WG : 1; }
+
+Phys make_phys(const pn1_ctx* c)
+{
+    Phys P;
+    P.eps = c->eps;
+    P.eps2 = c->eps * c->eps;
+    P.lam = 1.0 / (c->c * c->c);
+    P.dt = c->dt;
+    P.m0 = c->m0;
+    P.ws = 1.0 / std::sqrt(P.eps2);
+    P.eps2f = (float)P.eps2;
+    P.m0f = (float)c->m0;
+    P.n = c->n;
+    P.cap = c->cap;
+    return P;
+}
+
+// j split S and grid of the pair kernels for the current n.
+void plan(pn1_ctx* c)
+{
+    const int nst = c->n - 1;
+    c->nblk = nblocks(nst);
+    c->nIblk = nst > 0 ? (nst + WG - 1) / WG : 0;
+    int S = c->jsplit_req;
+    if (S <= 0) {
+        // enough workgroups to fill 256 CUs several times over, with
+        // chunks of at least 64 records
+        S = 1;
+        while (S < 64 && (long)S * c->nIblk < 2048 && (nst + 2 * S - 1) / (2 * S) >= 64) S *= 2;
+    }
+    if (nst > 0 && S > nst) S = nst;
+    if (S < 1) S = 1;
+    c->S = S;
+    if (S <= 8 && 8 % S == 0) {
+        const int per = 8 / S;
+        c->pair_grid = 8 * ((c->nIblk + per - 1) / per);
+    } else {
+        c->pair_grid = S * c->nIblk;
+    }
+    c->tile = c->tile_req > 0 ? c->tile_req : 32;
+}
+
+int ensure_part(pn1_ctx* c)
+{
+    const size_t cap = (size_t)c->cap, S = (size_t)c->S;
+    size_t need = S * NACC * cap * sizeof(double);
+    const size_t needE = S * NACCE * cap * sizeof(double);
+    const size_t need32 = S * NACC * 2 * cap * sizeof(float);
+    if (needE > need) need = needE;
+    if (need32 > need) need = need32;
+    if (need <= c->part_bytes) return PN1_OK;
+    if (c->part) hipFree(c->part);
+    c->part = nullptr;
+    c->part_bytes = 0;
+    if (hipMalloc(&c->part, need) != hipSuccess) {
+        c->part = nullptr;
+        return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes for pair partials failed", need);
+    }
+    c->part_bytes = need;
+    return PN1_OK;
+}
+
+void drop_graph(pn1_ctx* c)
+{
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+}
+
+bool ready(pn1_ctx* c, int* rc)
+{
+    if (!c) {
+        *rc = PN1_EINVAL;
+        return false;
+    }
+    if (!c->have_state) {
+        *rc = seterr(c, PN1_EINVAL, "no state uploaded");
+        return false;
+    }
+    if (hipSetDevice(c->dev) != hipSuccess) {
+        *rc = seterr(c, PN1_EDEVICE, "hipSetDevice(%d) failed", c->dev);
+        return false;
+    }
+    return true;
+}
+
+// ---- one force evaluation (+ stage update) ------------------------------
+template <int MODE, int STAGE>
+void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
+{
+    hipLaunchKernelGGL((k_fin<MODE, STAGE>), dim3(c->nblk), dim3(WG), 0, c->st, in, c->X, c->Y,
+                       c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P);
+    hipLaunchKernelGGL((k_bh<STAGE>), dim3(1), dim3(WG), 0, c->st, c->bhblk, c->nblk, c->gs, c->X,
+                       c->Y, c->H1, c->H2, c->acc, P);
+}
+
+// stage 0: plain evaluation on X; stages 1..3: the driver's force calls.
+int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
+{
+    const double* in = stage <= 1 ? c->X : c->Y;
+    const bool mixed = c->mode == PN1_MIXED;
+    hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
+                       mixed ? c->j32 : nullptr, c->gblk);
+    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
+    if (c->n > 1) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->prof) {
+            HIPCHK(c, hipEventCreate(&e0));
+            HIPCHK(c, hipEventCreate(&e1));
+            HIPCHK(c, hipEventRecord(e0, c->st));
+        }
+        if (mixed)
+            hipLaunchKernelGGL(k_pair32, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j32,
+                               (float*)c->part, P, c->S, c->nIblk, c->tile);
+        else
+            hipLaunchKernelGGL(k_pair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64,
+                               (double*)c->part, P, c->S, c->nIblk);
+        if (c->prof) {
+            HIPCHK(c, hipEventRecord(e1, c->st));
+            c->ev.emplace_back(e0, e1);
+        }
+    }
+    switch (stage * 2 + (mixed ? 1 : 0)) {
+    case 0: launch_fin<0, 0>(c, in, P); break;
+    case 1: launch_fin<1, 0>(c, in, P); break;
+    case 2: launch_fin<0, 1>(c, in, P); break;
+    case 3: launch_fin<1, 1>(c, in, P); break;
+    case 4: launch_fin<0, 2>(c, in, P); break;
+    case 5: launch_fin<1, 2>(c, in, P); break;
+    case 6: launch_fin<0, 3>(c, in, P); break;
+    default: launch_fin<1, 3>(c, in, P); break;
+    }
+    HIPCHK(c, hipGetLastError());
+    return PN1_OK;
+}
+
+int enqueue_step(pn1_ctx* c, const Phys& P)
+{
+    for (int stage = 1; stage <= 3; ++stage) {
+        int rc = enqueue_force(c, stage, P);
+        if (rc) return rc;
+    }
+    return PN1_OK;
+}
+
+// SoA device [k][cap] -> AoS host [n][3] for k in 0..2, through the stage.
+int fetch3(pn1_ctx* c, const double* dsrc, double* out)
+{
+    const size_t cap = (size_t)c->cap;
+    for (int k = 0; k < 3; ++k)
+        HIPCHK(c, hipMemcpyAsync(c->h_stage + k * cap, dsrc + k * cap, c->n * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int i = 0; i < c->n; ++i)
+        for (int k = 0; k < 3; ++k) out[3 * i + k] = c->h_stage[k * cap + i];
+    return PN1_OK;
+}
+
+}  // namespace
+
+// =========================================================================
+// context API
+// =========================================================================
+extern "C" {
+
+const char* pn1_last_error(const pn1_ctx* c) { return c ? c->err : "null context"; }
+
+void pn1_destroy(pn1_ctx* c)
+{
+    if (!c) return;
+    if (c->dev >= 0) hipSetDevice(c->dev);
+    if (c->st) hipStreamSynchronize(c->st);
+    drop_graph(c);
+    for (auto& e : c->ev) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    double* bufs[] = {c->X, c->Y, c->H1, c->H2, c->m, c->acc, c->gblk, c->gs, c->bhblk, c->eblk,
+                      c->eout};
+    for (double* b : bufs)
+        if (b) hipFree(b);
+    if (c->j64) hipFree(c->j64);
+    if (c->j32) hipFree(c->j32);
+    if (c->part) hipFree(c->part);
+    if (c->h_stage) hipHostFree(c->h_stage);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+}
+
+int pn1_create(pn1_ctx** out, int capacity, int device)
+{
+    if (!out) return PN1_EINVAL;
+    *out = nullptr;
+    if (capacity < 1) return PN1_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PN1_ENODEV;
+    if (device < 0 || device >= ndev) return PN1_ENODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PN1_ENODEV;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        std::fprintf(stderr, "libpn1: device %d is %s; this library is built for gfx950 only\n",
+                     device, prop.gcnArchName);
+        return PN1_ENODEV;
+    }
+    pn1_ctx* c = new pn1_ctx;
+    c->dev = device;
+    c->cap = capacity;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        pn1_destroy(c);
+        return PN1_EDEVICE;
+    }
+    const size_t cap = (size_t)capacity;
+    const size_t nb = (size_t)nblocks(capacity) + 1;
+    int rc = PN1_OK;
+    if (!rc) rc = dalloc(c, &c->X, 6 * cap);
+    if (!rc) rc = dalloc(c, &c->Y, 6 * cap);
+    if (!rc) rc = dalloc(c, &c->H1, 6 * cap);
+    if (!rc) rc = dalloc(c, &c->H2, 6 * cap);
+    if (!rc) rc = dalloc(c, &c->m, cap);
+    if (!rc) rc = dalloc(c, &c->acc, 3 * cap);
+    if (!rc) rc = dalloc(c, &c->gblk, nb * NGS);
+    if (!rc) rc = dalloc(c, &c->gs, 16);
+    if (!rc) rc = dalloc(c, &c->bhblk, nb * 3);
+    if (!rc) rc = dalloc(c, &c->eblk, nb * NEB);
+    if (!rc) rc = dalloc(c, &c->eout, 8);
+    if (!rc) rc = dalloc(c, &c->j64, cap);
+    if (!rc) rc = dalloc(c, &c->j32, cap);
+    if (!rc && hipHostMalloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
+        c->h_stage = nullptr;
+        rc = PN1_ENOMEM;
+    }
+    if (rc) {
+        pn1_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return PN1_OK;
+}
+
+int pn1_set_params(pn1_ctx* c, double eps, double cl, double dt, int mode)
+{
+    if (!c) return PN1_EINVAL;
+    if (!(eps > 0.0) || !std::isfinite(eps)) return seterr(c, PN1_EINVAL, "eps must be > 0");
+    if (!(cl > 0.0) || !std::isfinite(cl)) return seterr(c, PN1_EINVAL, "c must be > 0");
+    if (!(dt > 0.0) || !std::isfinite(dt)) return seterr(c, PN1_EINVAL, "dt must be > 0");
+    if (mode != PN1_FP64 && mode != PN1_MIXED) return seterr(c, PN1_EINVAL, "unknown mode %d", mode);
+    if (eps != c->eps || cl != c->c || dt != c->dt || mode != c->mode) drop_graph(c);
+    c->eps = eps;
+    c->c = cl;
+    c->dt = dt;
+    c->mode = mode;
+    return PN1_OK;
+}
+
+int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile)
+{
+    if (!c) return PN1_EINVAL;
+    if (jsplit < 0 || jsplit > 4096 || tile < 0 || tile > 65536)
+        return seterr(c, PN1_EINVAL, "bad tuning (jsplit %d, tile %d)", jsplit, tile);
+    c->jsplit_req = jsplit;
+    c->tile_req = tile;
+    drop_graph(c);
+    if (!c->have_state) return PN1_OK;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    plan(c);
+    return ensure_part(c);
+}
+
+int pn1_get_tuning(const pn1_ctx* c, int* jsplit, int* tile)
+{
+    if (!c) return PN1_EINVAL;
+    if (jsplit) *jsplit = c->S;
+    if (tile) *tile = c->tile;
+    return PN1_OK;
+}
+
+int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, int n)
+{
+    if (!c) return PN1_EINVAL;
+    char msg[256];
+    int rc = pn1_check_input(x, v, m, n, c->eps, msg, sizeof msg);
+    if (rc) return seterr(c, rc, "%s", msg);
+    if (n > c->cap) return seterr(c, PN1_EINVAL, "n = %d exceeds capacity %d", n, c->cap);
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const size_t cap = (size_t)c->cap;
+    double* h = c->h_stage;
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            h[k * cap + i] = x[3 * i + k];
+            h[(3 + k) * cap + i] = v[3 * i + k];
+        }
+        h[6 * cap + i] = m[i];
+    }
+    for (int k = 0; k < 6; ++k)
+        HIPCHK(c, hipMemcpyAsync(c->X + k * cap, h + k * cap, n * sizeof(double),
+                                 hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->m, h + 6 * cap, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (n != c->n || m[0] != c->m0) drop_graph(c);
+    c->n = n;
+    c->m0 = m[0];
+    c->have_state = true;
+    plan(c);
+    return ensure_part(c);
+}
+
+int pn1_download(pn1_ctx* c, double* x, double* v)
+{
+    int rc;
+    if (!ready(c, &rc)) return rc;
+    if (!x || !v) return seterr(c, PN1_EINVAL, "null output");
+    const size_t cap = (size_t)c->cap;
+    if ((rc = fetch3(c, c->X, x))) return rc;
+    return fetch3(c, c->X + 3 * cap, v);
+}
+
+int pn1_force(pn1_ctx* c, double* a)
+{
+    int rc;
+    if (!ready(c, &rc)) return rc;
+    if (!a) return seterr(c, PN1_EINVAL, "null output");
+    if ((rc = enqueue_force(c, 0, make_phys(c)))) return rc;
+    return fetch3(c, c->acc, a);
+}
+
+int pn1_energy(pn1_ctx* c, pn1_energy_t* out)
+{
+    int rc;
+    if (!ready(c, &rc)) return rc;
+    if (!out) return seterr(c, PN1_EINVAL, "null output");
+    const Phys P = make_phys(c);
+    hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, c->X, c->m, P, c->j64,
+                       (JRec32*)nullptr, c->gblk);
+    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
+    if (c->n > 1)
+        hipLaunchKernelGGL(k_epair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64,
+                           (double*)c->part, P, c->S, c->nIblk);
+    hipLaunchKernelGGL(k_efin, dim3(c->nblk), dim3(WG), 0, c->st, c->X, c->m,
+                       (const double*)c->part, c->n > 1 ? c->S : 0, c->gs, c->eblk, P);
+    hipLaunchKernelGGL(k_ered, dim3(1), dim3(WG), 0, c->st, c->eblk, c->nblk, c->X, c->m, c->gs,
+                       c->eout, P);
+    HIPCHK(c, hipGetLastError());
+    double e[5];
+    HIPCHK(c, hipMemcpyAsync(e, c->eout, sizeof e, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    out->kin = e[0];
+    out->pot = e[1];
+    out->pot_pn = e[2];
+    out->pot_pn2 = e[3];
+    out->total = e[4];
+    return PN1_OK;
+}
+
+int pn1_step(pn1_ctx* c, int steps)
+{
+    int rc;
+    if (!ready(c, &rc)) return rc;
+    if (steps < 0) return seterr(c, PN1_EINVAL, "steps < 0");
+    if (steps == 0) return PN1_OK;
+    const Phys P = make_phys(c);
+    if (c->prof) {
+        for (int s = 0; s < steps; ++s)
+            if ((rc = enqueue_step(c, P))) return rc;
+        return PN1_OK;
+    }
+    if (!c->gexec) {
+        hipGraph_t g = nullptr;
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        rc = enqueue_step(c, P);
+        hipError_t ec = hipStreamEndCapture(c->st, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess)
+            return seterr(c, PN1_EDEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->gexec = nullptr;
+            return seterr(c, PN1_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        }
+    }
+    for (int s = 0; s < steps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+    return PN1_OK;
+}
+
+int pn1_sync(pn1_ctx* c)
+{
+    if (!c) return PN1_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PN1_OK;
+}
+
+int pn1_profile_enable(pn1_ctx* c, int on)
+{
+    if (!c) return PN1_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (auto& e : c->ev) {
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    c->ev.clear();
+    c->prof = on != 0;
+    c->prof_launches = 0;
+    c->prof_ms = 0.0;
+    return PN1_OK;
+}
+
+int pn1_profile_read(pn1_ctx* c, int* launches, double* total_ms)
+{
+    if (!c) return PN1_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (auto& e : c->ev) {
+        float ms = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
+        c->prof_ms += ms;
+        c->prof_launches += 1;
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    c->ev.clear();
+    if (launches) *launches = c->prof_launches;
+    if (total_ms) *total_ms = c->prof_ms;
+    return PN1_OK;
+}
+
+// =========================================================================
+// legacy shapes
+// =========================================================================
+static pn1_ctx* g_ctx = nullptr;
+static double g_c = 1.0;
+static int g_mode = PN1_FP64;
+
+int pn1_set_c(double cl)
+{
+    if (!(cl > 0.0) || !std::isfinite(cl)) return PN1_EINVAL;
+    g_c = cl;
+    return PN1_OK;
+}
+
+int pn1_set_mode(int mode)
+{
+    if (mode != PN1_FP64 && mode != PN1_MIXED) return PN1_EINVAL;
+    g_mode = mode;
+    return PN1_OK;
+}
+
+static void die(const char* what, const char* msg)
+{
+    std::fprintf(stderr, "%s: %s\n", what, msg);
+    std::exit(1);
+}
+
+static pn1_ctx* legacy_ctx(const char* what, const double* x, const double* v, const double* m,
+                           int n, double eps, int mode)
+{
+    char msg[256];
+    if (pn1_check_input(x, v, m, n, eps, msg, sizeof msg) != PN1_OK) die(what, msg);
+    if (!g_ctx || g_ctx->cap < n) {
+        pn1_destroy(g_ctx);
+        g_ctx = nullptr;
+        int cap = n < 1024 ? 1024 : n;
+        int rc = pn1_create(&g_ctx, cap, 0);
+        if (rc != PN1_OK) {
+            std::snprintf(msg, sizeof msg, "cannot create a gfx950 device context (status %d)", rc);
+            die(what, msg);
+        }
+    }
+    if (pn1_set_params(g_ctx, eps, g_c, g_ctx->dt, mode) != PN1_OK ||
+        pn1_upload(g_ctx, x, v, m, n) != PN1_OK)
+        die(what, g_ctx->err);
+    return g_ctx;
+}
+
+void force(double (*x)[3], double (*v)[3], double* m, double eps, double (*a)[3], double* pot, int n)
+{
+    (void)pot;  // ignored, as in Limit-sticky9.c:21-424
+    pn1_ctx* c = legacy_ctx("force", &x[0][0], &v[0][0], m, n, eps, g_mode);
+    if (pn1_force(c, &a[0][0]) != PN1_OK) die("force", c->err);
+}
+
+static void legacy_energy(const char* what, double (*x)[3], double (*v)[3], double* m, int n,
+                          double eps, pn1_energy_t* e)
+{
+    pn1_ctx* c = legacy_ctx(what, &x[0][0], &v[0][0], m, n, eps, PN1_FP64);
+    if (pn1_energy(c, e) != PN1_OK) die(what, c->err);
+}
+
+// The print block of Limit-sticky9.c:560-563 / 701-703.
+static void print_block(double t, const pn1_energy_t* e)
+{
+    std::printf("time = %g\n", t);
+    std::printf("pot = %22.15e kin = %22.15e \n pot_pn = %22.15e \n  total= %22.15e ratio = %e\n",
+                e->pot, e->kin, e->pot_pn, e->total, e->kin / e->pot);
+}
+
+void energy(double t, double (*x)[3], double (*v)[3], double* m, int n, double init_ene, double eps)
+{
+    pn1_energy_t e;
+    legacy_energy("energy", x, v, m, n, eps, &e);
+    const double de = (init_ene - e.total) / e.total;
+    print_block(t, &e);
+    std::printf(" DE = %e %g\n", de, t);
+}
+
+void initial_energy(double (*x)[3], double (*v)[3], double* m, int n, double* init_ene, double eps)
+{
+    pn1_energy_t e;
+    legacy_energy("initial_energy", x, v, m, n, eps, &e);
+    print_block(0.0, &e);
+    *init_ene = e.total;
+}
+
+}  // extern "C"

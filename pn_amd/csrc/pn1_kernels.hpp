@@ -1,0 +1,719 @@
+// pn1_kernels.hpp -- device code of libpn1 (gfx950, wave64).
+//
+// One force evaluation of Limit-sticky9's force() (Limit-sticky9.c:21-424)
+// is five launches on one stream:
+//
+//   k_prep    O(n)   per-star BH geometry (d, |d|^2, e = (|d|^2+eps^2)^-1/2)
+//                    written once as an AoS j-record, plus per-block
+//                    partials of the separable sums (sum m e, sum m e^3 d, ...)
+//   k_gsum    1 WG   fixed-order reduction of those partials
+//   k_pair*   O(n^2) the star-star pair sums, one rsqrt per pair
+//   k_fin     O(n)   star accelerations from the pair sums and the O(n)
+//                    closed forms; the stage update of the RK driver; per-block
+//                    partials of the BH's 1PN acceleration
+//   k_bh      1 WG   BH acceleration and its stage update
+//
+// Pair kernels: i-particle per lane in VGPRs; the j loop has wave-uniform
+// bounds, so each j-record is fetched with scalar loads into SGPRs and
+// broadcast to all 64 lanes as an instruction operand (no LDS traffic in
+// the inner loop).  The j range is split into S chunks (one partial per
+// chunk, reduced in fixed order by k_fin), and workgroups are mapped so that
+// each XCD works on one chunk, whose records then stay in that XCD's L2.
+//
+// All reductions use a fixed order, so results are bitwise reproducible
+// for a given launch shape.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pn1dev {
+
+constexpr int WG = 256;     // threads per workgroup (4 waves of 64)
+constexpr int NACC = 15;    // force pair accumulators
+constexpr int NACCE = 5;    // energy pair accumulators
+constexpr int NGS = 7;      // separable O(n) sums
+constexpr int NEB = 4;      // energy per-block partials
+
+// force pair accumulators (per star i, sums over stars j != i; t = |r|^2 - D2_i):
+//   N    = sum m_j w^3 r                                      (Newton)
+//   Rr   = sum K_r r   K_r = m_j w^3 [m0(4e_i + 1.25e_j + 0.25e_j^3 t)
+//                                       + |v_i|^2 - 2|u|^2 + 1.5 (v_j.r)^2 w^2]
+//   Ru   = sum m_j w^3 (r.(4v_i - 3v_j)) u
+//   T3   = sum 3.5 m0 m_j e_j^3 (w - e_i) d_j
+//   phi  = sum m_j w,  xi = sum m_j w^3 (D2_i - D2_j),  zeta = sum m_j 0.25 m0 e_j^3 t
+// T3 keeps the reference's grouping (w - e_i) d_j (Limit-sticky9.c:307-308),
+// and zeta shares t with K_r, as the reference's two e_j^3 terms share
+// r1b2 - rbc2 (:302, 305): both pairs of terms cancel strongly when star j
+// is close to the BH, and splitting them differently loses digits.
+enum { A_NX, A_NY, A_NZ, A_RX, A_RY, A_RZ, A_UX, A_UY, A_UZ, A_TX, A_TY, A_TZ,
+       A_PHI, A_XI, A_ZE };
+// energy pair accumulators: phi, sum m_j w v_j, sum m_j w^3 (r.v_i)(r.v_j)
+enum { E_PHI, E_PVX, E_PVY, E_PVZ, E_GAM };
+// separable sums over stars: sum m e, sum m e^3 d, sum m v
+enum { G_G1, G_M1X, G_M1Y, G_M1Z, G_PX, G_PY, G_PZ };
+
+// j-record, FP64 form: position, velocity, mass, |d|^2, 1.25 m0 e,
+// 0.25 m0 e^3, 3.5 m0 m e^3, e, d = x - x_BH, m 0.25 m0 e^3.  128 bytes =
+// two 16-dword scalar loads.
+struct __align__(16) JRec64 {
+    double x, y, z, vx, vy, vz, m, D2, A, B, Q, e, dx, dy, dz, MB;
+};
+// j-record, mixed form: double-single position (hi, lo), the rest rounded
+// to FP32.  80 bytes.
+struct __align__(16) JRec32 {
+    float xh, yh, zh, xl, yl, zl, vx, vy, vz, m, D2, A, B, Q, e, dx, dy, dz, MB, pad;
+};
+static_assert(sizeof(JRec64) == 128, "JRec64 layout");
+static_assert(sizeof(JRec32) == 80, "JRec32 layout");
+
+// Everything a kernel needs about the run, passed by value.
+struct Phys {
+    double eps, eps2, lam, dt, m0;
+    double ws;       // 1/sqrt(eps^2): the reference's self-pair w
+    float eps2f, m0f;
+    int n, cap;      // particles incl. the BH; SoA stride
+};
+
+// ---------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------
+// FP64 reciprocal square root.  v_rsq_f64 alone is good to ~2^-24
+// relative (measured on gfx950), so one correction of third order follows:
+// with r = 1 - x y^2, 1/sqrt(x) = y (1 - r)^(-1/2) = y (1 + r/2 + 3r^2/8 + ...),
+// truncation ~ (5/16) r^3 ~ 1e-22; the result is within ~1 ulp.
+__device__ __forceinline__ double rsq64(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double h = x * y;
+    const double r = __builtin_fma(-h, y, 1.0);
+    const double t = __builtin_fma(0.375, r, 0.5);
+    return __builtin_fma(y * r, t, y);
+}
+
+// Fixed-order workgroup sum of N doubles; thread 0 writes out[0..N).
+// Wave level: xor butterfly (every lane ends with the same bits); then the
+// four wave totals are added in wave order.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* out)
+{
+    __shared__ double sh[N][WG / 64];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double t = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        v[k] = t;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[k][wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    }
+}
+
+// Workgroup -> (j chunk s, i block ib).  When S divides 8, consecutive
+// workgroup ids (which the dispatcher deals round-robin to the 8 XCDs) are
+// arranged so that every workgroup on XCD x works on chunk x % S.
+__device__ __forceinline__ void map_block(int bid, int S, int& s, int& ib)
+{
+    if (S <= 8 && (8 % S) == 0) {
+        const int xcd = bid & 7, k = bid >> 3, per = 8 / S;
+        s = xcd % S;
+        ib = k * per + xcd / S;
+    } else {
+        s = bid % S;
+        ib = bid / S;
+    }
+}
+
+// Stage update of the reference's RK driver for particle p, given its
+// acceleration a (Limit-sticky9.c:853-891).  STAGE 1..3 follow the three
+// force calls; STAGE 0 (plain force evaluation) does nothing.  FMA
+// contraction is off so each line rounds exactly as the reference's.
+template <int STAGE>
+__device__ __forceinline__ void stage_update(int p, int cap, const double (&a)[3],
+                                             double* X, double* Y, double* H1,
+                                             double* H2, double dt)
+{
+#pragma clang fp contract(off)
+    if (STAGE == 0) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const size_t ix = (size_t)k * cap + p, iv = (size_t)(3 + k) * cap + p;
+        const double x = X[ix], v = X[iv];
+        if (STAGE == 1) {
+            const double x1 = x + 0.5 * v * dt;
+            const double v1 = v + 0.5 * a[k] * dt;
+            Y[ix] = x1; Y[iv] = v1; H1[ix] = x1; H1[iv] = v1;
+        } else if (STAGE == 2) {
+            const double v1o = Y[iv];            // v1 of stage 1
+            const double x1 = x + 0.5 * v1o * dt;
+            const double v1 = v + 0.5 * a[k] * dt;
+            Y[ix] = x1; Y[iv] = v1; H2[ix] = x1; H2[iv] = v1;
+        } else {
+            const double v1o = Y[iv];            // v1 of stage 2
+            const double hx3 = x + v1o * dt;
+            const double hv3 = v + a[k] * dt;
+            X[ix] = (x + hx3 + 2.0 * (H1[ix] + H2[ix])) / 6.0;
+            X[iv] = (v + hv3 + 2.0 * (H1[iv] + H2[iv])) / 6.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// k_prep: per-star BH geometry + separable sums
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
+                                            const double* __restrict__ m, Phys P,
+                                            JRec64* __restrict__ j64,
+                                            JRec32* __restrict__ j32,
+                                            double* __restrict__ gblk)
+{
+#pragma clang fp contract(off)
+    const int p = blockIdx.x * WG + threadIdx.x + 1;
+    const int cap = P.cap;
+    double g[NGS];
+#pragma unroll
+    for (int k = 0; k < NGS; ++k) g[k] = 0.0;
+    if (p < P.n) {
+        const double x = st[p], y = st[cap + p], z = st[2 * cap + p];
+        const double vx = st[3 * cap + p], vy = st[4 * cap + p], vz = st[5 * cap + p];
+        const double mp = m[p];
+        const double dx = x - st[0], dy = y - st[cap], dz = z - st[2 * cap];
+        const double D2 = dx * dx + dy * dy + dz * dz;
+        const double e = 1.0 / sqrt(D2 + P.eps2);     // as Limit-sticky9.c:106-108
+        const double E3 = e * e * e;
+        const double q = mp * E3;
+        JRec64 r;
+        r.x = x; r.y = y; r.z = z; r.vx = vx; r.vy = vy; r.vz = vz;
+        r.m = mp; r.D2 = D2;
+        r.A = 1.25 * P.m0 * e; r.B = 0.25 * P.m0 * E3; r.Q = 3.5 * P.m0 * q; r.e = e;
+        r.dx = dx; r.dy = dy; r.dz = dz; r.MB = mp * r.B;
+        j64[p] = r;
+        if (j32) {
+            JRec32 h;
+            h.xh = (float)x; h.yh = (float)y; h.zh = (float)z;
+            h.xl = (float)(x - (double)h.xh);
+            h.yl = (float)(y - (double)h.yh);
+            h.zl = (float)(z - (double)h.zh);
+            h.vx = (float)vx; h.vy = (float)vy; h.vz = (float)vz;
+            h.m = (float)mp; h.D2 = (float)D2;
+            h.A = (float)r.A; h.B = (float)r.B; h.Q = (float)r.Q; h.e = (float)e;
+            h.dx = (float)dx; h.dy = (float)dy; h.dz = (float)dz; h.MB = (float)r.MB;
+            h.pad = 0.0f;
+            j32[p] = h;
+        }
+        g[G_G1] = mp * e;
+        g[G_M1X] = q * dx; g[G_M1Y] = q * dy; g[G_M1Z] = q * dz;
+        g[G_PX] = mp * vx; g[G_PY] = mp * vy; g[G_PZ] = mp * vz;
+    }
+    block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
+}
+
+// Fixed-order sum of nblk rows of N doubles (one workgroup).
+template <int N>
+__global__ __launch_bounds__(WG) void k_rows_sum(const double* __restrict__ rows, int nblk,
+                                                double* __restrict__ out)
+{
+    double g[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) g[k] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += WG)
+#pragma unroll
+        for (int k = 0; k < N; ++k) g[k] += rows[(size_t)b * N + k];
+    block_sum<N>(g, out);
+}
+
+// ---------------------------------------------------------------------
+// force pair kernel, FP64
+// ---------------------------------------------------------------------
+struct I64 {
+    double x, y, z, vx, vy, vz, v4x, v4y, v4z, al, D2, D2e, e;
+};
+
+__device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr, int i, double m0,
+                                        double eps2)
+{
+    const JRec64 r = jr[i];
+    I64 I;
+    I.x = r.x; I.y = r.y; I.z = r.z;
+    I.vx = r.vx; I.vy = r.vy; I.vz = r.vz;
+    I.v4x = 4.0 * r.vx; I.v4y = 4.0 * r.vy; I.v4z = 4.0 * r.vz;
+    const double V2 = r.vx * r.vx + r.vy * r.vy + r.vz * r.vz;
+    I.al = 4.0 * m0 * r.e + V2;
+    I.D2 = r.D2;
+    I.D2e = r.D2 + eps2;
+    I.e = r.e;
+    return I;
+}
+
+template <bool MASK>
+__device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64& I,
+                                       double (&a)[NACC], double eps2)
+{
+    double mj = J.m, Qj = J.Q;
+    if (MASK) {
+        const bool self = (j == i);
+        mj = self ? 0.0 : mj;
+        Qj = self ? 0.0 : Qj;
+    }
+    const double rx = J.x - I.x, ry = J.y - I.y, rz = J.z - I.z;
+    const double R2e = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
+    const double w = rsq64(R2e);
+    const double w2 = w * w;
+    const double mw = mj * w;
+    const double mw3 = mw * w2;
+    a[A_NX] = __builtin_fma(mw3, rx, a[A_NX]);
+    a[A_NY] = __builtin_fma(mw3, ry, a[A_NY]);
+    a[A_NZ] = __builtin_fma(mw3, rz, a[A_NZ]);
+    a[A_PHI] += mw;
+    a[A_XI] = __builtin_fma(mw3, I.D2 - J.D2, a[A_XI]);
+    const double c3 = Qj * (w - I.e);
+    a[A_TX] = __builtin_fma(c3, J.dx, a[A_TX]);
+    a[A_TY] = __builtin_fma(c3, J.dy, a[A_TY]);
+    a[A_TZ] = __builtin_fma(c3, J.dz, a[A_TZ]);
+    const double t = R2e - I.D2e;
+    a[A_ZE] = __builtin_fma(mj * J.B, t, a[A_ZE]);
+    const double ux = J.vx - I.vx, uy = J.vy - I.vy, uz = J.vz - I.vz;
+    const double uu = __builtin_fma(ux, ux, __builtin_fma(uy, uy, uz * uz));
+    const double vjr = __builtin_fma(J.vx, rx, __builtin_fma(J.vy, ry, J.vz * rz));
+    double br = __builtin_fma(J.B, t, I.al + J.A);
+    br = __builtin_fma(-2.0, uu, br);
+    const double tv = vjr * w;
+    br = __builtin_fma(1.5 * tv, tv, br);
+    const double Kr = mw3 * br;
+    a[A_RX] = __builtin_fma(Kr, rx, a[A_RX]);
+    a[A_RY] = __builtin_fma(Kr, ry, a[A_RY]);
+    a[A_RZ] = __builtin_fma(Kr, rz, a[A_RZ]);
+    const double v4r = __builtin_fma(I.v4x, rx, __builtin_fma(I.v4y, ry, I.v4z * rz));
+    const double Ku = mw3 * __builtin_fma(-3.0, vjr, v4r);
+    a[A_UX] = __builtin_fma(Ku, ux, a[A_UX]);
+    a[A_UY] = __builtin_fma(Ku, uy, a[A_UY]);
+    a[A_UZ] = __builtin_fma(Ku, uz, a[A_UZ]);
+}
+
+// Chunk bounds of the j range [1, n) split in S parts, and the part of it
+// that can contain j == i for this wave's 64 lanes.
+struct JRange {
+    int j0, j1, w0, w1;
+};
+__device__ __forceinline__ JRange j_range(int s, int S, int n, int ib)
+{
+    JRange R;
+    const int nst = n - 1;
+    const int clen = (nst + S - 1) / S;
+    R.j0 = 1 + s * clen;
+    R.j1 = min(n, R.j0 + clen);
+    const int wb = ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63) + 1;
+    R.w0 = max(R.j0, wb);
+    R.w1 = min(R.j1, wb + 64);
+    return R;
+}
+
+__global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
+                                              double* __restrict__ part, Phys P, int S,
+                                              int nIblk)
+{
+    int s, ib;
+    map_block(blockIdx.x, S, s, ib);
+    if (ib >= nIblk) return;
+    const int n = P.n;
+    const int li = ib * WG + threadIdx.x + 1;
+    const int i = li < n ? li : n - 1;
+    const I64 I = load_i64(jr, i, P.m0, P.eps2);
+    const JRange R = j_range(s, S, n, ib);
+    const double eps2 = P.eps2;
+    double a[NACC];
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+    if (R.w0 < R.w1) {
+        for (int j = R.j0; j < R.w0; ++j) body64<false>(jr[j], j, i, I, a, eps2);
+        for (int j = R.w0; j < R.w1; ++j) body64<true>(jr[j], j, i, I, a, eps2);
+        for (int j = R.w1; j < R.j1; ++j) body64<false>(jr[j], j, i, I, a, eps2);
+    } else {
+        for (int j = R.j0; j < R.j1; ++j) body64<false>(jr[j], j, i, I, a, eps2);
+    }
+    if (li < n) {
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) part[(size_t)(s * NACC + k) * P.cap + li] = a[k];
+    }
+}
+
+// ---------------------------------------------------------------------
+// force pair kernel, mixed: FP32 pair arithmetic on double-single
+// positions, FP32 tile sums folded into Kahan-compensated FP32 totals
+// ---------------------------------------------------------------------
+struct I32 {
+    float xh, yh, zh, xl, yl, zl, vx, vy, vz, v4x, v4y, v4z, al, D2, D2e, e;
+};
+
+__device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr, int i, float m0,
+                                        float eps2)
+{
+    const JRec32 r = jr[i];
+    I32 I;
+    I.xh = r.xh; I.yh = r.yh; I.zh = r.zh;
+    I.xl = r.xl; I.yl = r.yl; I.zl = r.zl;
+    I.vx = r.vx; I.vy = r.vy; I.vz = r.vz;
+    I.v4x = 4.0f * r.vx; I.v4y = 4.0f * r.vy; I.v4z = 4.0f * r.vz;
+    const float V2 = r.vx * r.vx + r.vy * r.vy + r.vz * r.vz;
+    I.al = 4.0f * m0 * r.e + V2;
+    I.D2 = r.D2;
+    I.D2e = r.D2 + eps2;
+    I.e = r.e;
+    return I;
+}
+
+template <bool MASK>
+__device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32& I,
+                                       float (&a)[NACC], float eps2)
+{
+    float mj = J.m, Qj = J.Q;
+    if (MASK) {
+        const bool self = (j == i);
+        mj = self ? 0.0f : mj;
+        Qj = self ? 0.0f : Qj;
+    }
+    // separation from the hi and lo parts: close pairs keep their digits
+    const float rx = (J.xh - I.xh) + (J.xl - I.xl);
+    const float ry = (J.yh - I.yh) + (J.yl - I.yl);
+    const float rz = (J.zh - I.zh) + (J.zl - I.zl);
+    const float R2e = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+    const float w = __builtin_amdgcn_rsqf(R2e);
+    const float w2 = w * w;
+    const float mw = mj * w;
+    const float mw3 = mw * w2;
+    a[A_NX] = __builtin_fmaf(mw3, rx, a[A_NX]);
+    a[A_NY] = __builtin_fmaf(mw3, ry, a[A_NY]);
+    a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
+    a[A_PHI] += mw;
+    a[A_XI] = __builtin_fmaf(mw3, I.D2 - J.D2, a[A_XI]);
+    const float c3 = Qj * (w - I.e);
+    a[A_TX] = __builtin_fmaf(c3, J.dx, a[A_TX]);
+    a[A_TY] = __builtin_fmaf(c3, J.dy, a[A_TY]);
+    a[A_TZ] = __builtin_fmaf(c3, J.dz, a[A_TZ]);
+    const float t = R2e - I.D2e;
+    a[A_ZE] = __builtin_fmaf(mj * J.B, t, a[A_ZE]);
+    const float ux = J.vx - I.vx, uy = J.vy - I.vy, uz = J.vz - I.vz;
+    const float uu = __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz));
+    const float vjr = __builtin_fmaf(J.vx, rx, __builtin_fmaf(J.vy, ry, J.vz * rz));
+    float br = __builtin_fmaf(J.B, t, I.al + J.A);
+    br = __builtin_fmaf(-2.0f, uu, br);
+    const float tv = vjr * w;
+    br = __builtin_fmaf(1.5f * tv, tv, br);
+    const float Kr = mw3 * br;
+    a[A_RX] = __builtin_fmaf(Kr, rx, a[A_RX]);
+    a[A_RY] = __builtin_fmaf(Kr, ry, a[A_RY]);
+    a[A_RZ] = __builtin_fmaf(Kr, rz, a[A_RZ]);
+    const float v4r = __builtin_fmaf(I.v4x, rx, __builtin_fmaf(I.v4y, ry, I.v4z * rz));
+    const float Ku = mw3 * __builtin_fmaf(-3.0f, vjr, v4r);
+    a[A_UX] = __builtin_fmaf(Ku, ux, a[A_UX]);
+    a[A_UY] = __builtin_fmaf(Ku, uy, a[A_UY]);
+    a[A_UZ] = __builtin_fmaf(Ku, uz, a[A_UZ]);
+}
+
+// Kahan step: s + c carries the running total, c the negated lost part.
+__device__ __forceinline__ void kahan_add(float& s, float& c, float t)
+{
+#pragma clang fp contract(off)
+    const float y = t - c;
+    const float u = s + y;
+    c = (u - s) - y;
+    s = u;
+}
+
+__global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
+                                              float* __restrict__ part, Phys P, int S,
+                                              int nIblk, int tile)
+{
+    int s, ib;
+    map_block(blockIdx.x, S, s, ib);
+    if (ib >= nIblk) return;
+    const int n = P.n;
+    const int li = ib * WG + threadIdx.x + 1;
+    const int i = li < n ? li : n - 1;
+    const I32 I = load_i32(jr, i, P.m0f, P.eps2f);
+    const JRange R = j_range(s, S, n, ib);
+    const float eps2 = P.eps2f;
+    float sum[NACC], cmp[NACC];
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) sum[k] = cmp[k] = 0.0f;
+    for (int jt = R.j0; jt < R.j1; jt += tile) {
+        const int je = min(jt + tile, R.j1);
+        float a[NACC];
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
+        if (jt < R.w1 && je > R.w0) {
+            for (int j = jt; j < je; ++j) body32<true>(jr[j], j, i, I, a, eps2);
+        } else {
+            for (int j = jt; j < je; ++j) body32<false>(jr[j], j, i, I, a, eps2);
+        }
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) kahan_add(sum[k], cmp[k], a[k]);
+    }
+    if (li < n) {
+        const size_t cap = (size_t)P.cap;
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) {
+            part[((size_t)(s * NACC + k) * 2 + 0) * cap + li] = sum[k];
+            part[((size_t)(s * NACC + k) * 2 + 1) * cap + li] = cmp[k];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// k_fin: star accelerations, stage update, BH partials
+// ---------------------------------------------------------------------
+template <int MODE>
+__device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S, int cap, int p,
+                                           double (&A)[NACC])
+{
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) A[k] = 0.0;
+    if (MODE == 0) {
+        const double* pp = (const double*)part;
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) A[k] += pp[(size_t)(s * NACC + k) * cap + p];
+    } else {
+        const float* pp = (const float*)part;
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) {
+                const double hi = pp[((size_t)(s * NACC + k) * 2 + 0) * cap + p];
+                const double lo = pp[((size_t)(s * NACC + k) * 2 + 1) * cap + p];
+                A[k] += hi - lo;
+            }
+    }
+}
+
+template <int MODE, int STAGE>
+__global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double* Y, double* H1,
+                                           double* H2, const double* __restrict__ m,
+                                           const void* __restrict__ part, int S,
+                                           const double* __restrict__ gs,
+                                           double* __restrict__ acc,
+                                           double* __restrict__ bhblk, Phys P)
+{
+    const int p = blockIdx.x * WG + threadIdx.x + 1;
+    const int cap = P.cap;
+    double cb[3] = {0.0, 0.0, 0.0};
+    if (p < P.n) {
+        const double m0 = P.m0, mp = m[p], eps2 = P.eps2;
+        const double x[3] = {in[p], in[cap + p], in[2 * cap + p]};
+        const double v[3] = {in[3 * cap + p], in[4 * cap + p], in[5 * cap + p]};
+        const double x0[3] = {in[0], in[cap], in[2 * cap]};
+        double d[3], D2, e;
+        {
+#pragma clang fp contract(off)
+            for (int k = 0; k < 3; ++k) d[k] = x[k] - x0[k];
+            D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            e = 1.0 / sqrt(D2 + eps2);
+        }
+        const double E3 = e * e * e, q = mp * E3;
+        double S_[NACC];
+        gather_acc<MODE>(part, S, cap, p, S_);
+        const double G1 = gs[G_G1];
+        const double Pm[3] = {gs[G_PX], gs[G_PY], gs[G_PZ]};
+        const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        const double vd = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+
+        // ---- star i: Newton + B + S + sum over j != i of the ac pair term
+        const double Pp[3] = {Pm[0] - mp * v[0], Pm[1] - mp * v[1], Pm[2] - mp * v[2]};
+        const double vP = v[0] * Pp[0] + v[1] * Pp[1] + v[2] * Pp[2];
+        const double Pd = Pp[0] * d[0] + Pp[1] * d[1] + Pp[2] * d[2];
+        const double c1 = -m0 * E3 * (4.0 * S_[A_PHI] + 1.25 * (G1 - mp * e)) + E3 * S_[A_ZE];
+        const double cB = 4.0 * m0 * e - V2;
+        const double cS = 5.0 * m0 * mp * e * e * e * e - 4.0 * mp * E3 * V2;
+        double a[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double newton = S_[A_NX + k] - m0 * E3 * d[k];
+            const double B = -m0 * E3 * (d[k] * cB + 4.0 * vd * v[k]);
+            const double Sg = cS * d[k] + 7.0 * mp * E3 * vd * v[k];
+            const double t4 = E3 * (4.0 * vP * d[k] - 3.0 * Pd * v[k] - 4.0 * vd * Pp[k]);
+            const double pn = B + Sg + S_[A_RX + k] + S_[A_UX + k] + c1 * d[k] + S_[A_TX + k] + t4;
+            a[k] = newton + P.lam * pn;
+            acc[(size_t)k * cap + p] = a[k];
+        }
+
+        // ---- this star's share of the BH's 1PN acceleration: a1 + a1c
+        // the j = i bracket of the a1c pair term in the reference's
+        // operation order (Limit-sticky9.c:139-140 at |r|^2 = 0)
+        double self_br;
+        {
+#pragma clang fp contract(off)
+            const double ws = P.ws;
+            self_br = 4.0 * e + 1.25 * ws - 0.25 * D2 / eps2 * ws + 0.25 * D2 / eps2 * ws;
+        }
+        const double vP0 = v[0] * Pm[0] + v[1] * Pm[1] + v[2] * Pm[2];
+        const double Pd0 = Pm[0] * d[0] + Pm[1] * d[1] + Pm[2] * d[2];
+        const double mE3 = mp * E3;
+        const double cA1 = mE3 * (5.0 * m0 * e - 2.0 * V2 + 1.5 * vd * vd / D2);
+        const double cC0 = 4.0 * mp * mp * e * e * e * e;
+        const double cC2 = mE3 * (4.0 * (G1 - mp * e) + 1.25 * S_[A_PHI] + 0.25 * S_[A_XI]
+                                  + mp * self_br);
+        const double cQ2 = mp * mp * E3 * (4.0 * e + 1.25 * P.eps);
+        const double vm = mE3 / m0, vm2 = mp * mp * E3 / m0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double A1 = cA1 * d[k] + 3.0 * mE3 * vd * v[k];
+            const double C2 = cC2 * d[k] - 3.5 * e * mp * S_[A_NX + k]
+                              - vm * (4.0 * vP0 * d[k] - 3.0 * vd * Pm[k] - 4.0 * Pd0 * v[k]);
+            const double Q2 = cQ2 * d[k] - vm2 * (4.0 * V2 * d[k] - 7.0 * vd * v[k]);
+            cb[k] = A1 + cC0 * d[k] + C2 + Q2;
+        }
+        stage_update<STAGE>(p, cap, a, X, Y, H1, H2, P.dt);
+    }
+    block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
+}
+
+// BH: Newton sum_j m_j e_j^3 d_j plus lam times the star shares.
+template <int STAGE>
+__global__ __launch_bounds__(WG) void k_bh(const double* __restrict__ bhblk, int nblk,
+                                          const double* __restrict__ gs, double* X, double* Y,
+                                          double* H1, double* H2, double* __restrict__ acc,
+                                          Phys P)
+{
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += WG)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s[k] += bhblk[(size_t)b * 3 + k];
+    __shared__ double tot[3];
+    block_sum<3>(s, tot);
+    if (threadIdx.x == 0) {
+        double a0[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            a0[k] = gs[G_M1X + k] + P.lam * tot[k];
+            acc[(size_t)k * P.cap] = a0[k];
+        }
+        stage_update<STAGE>(0, P.cap, a0, X, Y, H1, H2, P.dt);
+    }
+}
+
+// ---------------------------------------------------------------------
+// energy: pair sums (FP64 in both modes), per-star terms, totals
+// ---------------------------------------------------------------------
+template <bool MASK>
+__device__ __forceinline__ void ebody64(const JRec64& J, int j, int i, const I64& I,
+                                        double (&a)[NACCE], double eps2)
+{
+    double mj = J.m;
+    if (MASK) mj = (j == i) ? 0.0 : mj;
+    const double rx = J.x - I.x, ry = J.y - I.y, rz = J.z - I.z;
+    const double R2e = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
+    const double w = rsq64(R2e);
+    const double mw = mj * w;
+    a[E_PHI] += mw;
+    a[E_PVX] = __builtin_fma(mw, J.vx, a[E_PVX]);
+    a[E_PVY] = __builtin_fma(mw, J.vy, a[E_PVY]);
+    a[E_PVZ] = __builtin_fma(mw, J.vz, a[E_PVZ]);
+    const double rvi = __builtin_fma(I.vx, rx, __builtin_fma(I.vy, ry, I.vz * rz));
+    const double rvj = __builtin_fma(J.vx, rx, __builtin_fma(J.vy, ry, J.vz * rz));
+    a[E_GAM] = __builtin_fma(mw * (w * w), rvi * rvj, a[E_GAM]);
+}
+
+__global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
+                                               double* __restrict__ part, Phys P, int S,
+                                               int nIblk)
+{
+    int s, ib;
+    map_block(blockIdx.x, S, s, ib);
+    if (ib >= nIblk) return;
+    const int n = P.n;
+    const int li = ib * WG + threadIdx.x + 1;
+    const int i = li < n ? li : n - 1;
+    const I64 I = load_i64(jr, i, P.m0, P.eps2);
+    const JRange R = j_range(s, S, n, ib);
+    const double eps2 = P.eps2;
+    double a[NACCE];
+#pragma unroll
+    for (int k = 0; k < NACCE; ++k) a[k] = 0.0;
+    if (R.w0 < R.w1) {
+        for (int j = R.j0; j < R.w0; ++j) ebody64<false>(jr[j], j, i, I, a, eps2);
+        for (int j = R.w0; j < R.w1; ++j) ebody64<true>(jr[j], j, i, I, a, eps2);
+        for (int j = R.w1; j < R.j1; ++j) ebody64<false>(jr[j], j, i, I, a, eps2);
+    } else {
+        for (int j = R.j0; j < R.j1; ++j) ebody64<false>(jr[j], j, i, I, a, eps2);
+    }
+    if (li < n) {
+#pragma unroll
+        for (int k = 0; k < NACCE; ++k) part[(size_t)(s * NACCE + k) * P.cap + li] = a[k];
+    }
+}
+
+// Per-star energy terms of Limit-sticky9.c:452-550 with the self pair
+// excluded; per-block partials {sum m v^2, -1/2 sum m phi, pot_pn, pot_pn2}.
+__global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
+                                            const double* __restrict__ m,
+                                            const double* __restrict__ part, int S,
+                                            const double* __restrict__ gs,
+                                            double* __restrict__ eblk, Phys P)
+{
+    const int p = blockIdx.x * WG + threadIdx.x + 1;
+    const int cap = P.cap;
+    double t[NEB] = {0.0, 0.0, 0.0, 0.0};
+    if (p < P.n) {
+        const double m0 = P.m0, mp = m[p];
+        const double x[3] = {st[p], st[cap + p], st[2 * cap + p]};
+        const double v[3] = {st[3 * cap + p], st[4 * cap + p], st[5 * cap + p]};
+        const double v0[3] = {st[3 * cap], st[4 * cap], st[5 * cap]};
+        double d[3], e;
+        {
+#pragma clang fp contract(off)
+            for (int k = 0; k < 3; ++k) d[k] = x[k] - st[(size_t)k * cap];
+            e = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + P.eps2);
+        }
+        double A[NACCE] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int k = 0; k < NACCE; ++k) A[k] += part[(size_t)(s * NACCE + k) * cap + p];
+        const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        const double v0v = v0[0] * v[0] + v0[1] * v[1] + v0[2] * v[2];
+        const double dv0 = d[0] * v0[0] + d[1] * v0[1] + d[2] * v0[2];
+        const double dv = d[0] * v[0] + d[1] * v[1] + d[2] * v[2];
+        const double vPhv = v[0] * A[E_PVX] + v[1] * A[E_PVY] + v[2] * A[E_PVZ];
+        const double G1 = gs[G_G1];
+        t[0] = mp * V2;
+        t[1] = -0.5 * mp * A[E_PHI];
+        t[2] = 0.375 * mp * V2 * V2 + 1.5 * m0 * e * mp * V2 + 0.5 * m0 * m0 * e * e * mp
+               + 0.5 * m0 * e * mp * (3.0 * V2 - 7.0 * v0v - dv0 * dv * e * e);
+        t[3] = mp * (0.25 * (6.0 * V2 * A[E_PHI] - 7.0 * vPhv - A[E_GAM]) + m0 * e * A[E_PHI]
+                     + 0.5 * m0 * e * (G1 - mp * e));
+    }
+    block_sum<NEB>(t, eblk + (size_t)blockIdx.x * NEB);
+}
+
+// out = {kin, pot, pot_pn/c^2, pot_pn2/c^2, total}
+__global__ __launch_bounds__(WG) void k_ered(const double* __restrict__ eblk, int nblk,
+                                            const double* __restrict__ st,
+                                            const double* __restrict__ m,
+                                            const double* __restrict__ gs,
+                                            double* __restrict__ out, Phys P)
+{
+    double s[NEB] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += WG)
+#pragma unroll
+        for (int k = 0; k < NEB; ++k) s[k] += eblk[(size_t)b * NEB + k];
+    __shared__ double tot[NEB];
+    block_sum<NEB>(s, tot);
+    if (threadIdx.x == 0) {
+        const int cap = P.cap;
+        const double v0x = st[3 * cap], v0y = st[4 * cap], v0z = st[5 * cap];
+        const double kin = 0.5 * (m[0] * (v0x * v0x + v0y * v0y + v0z * v0z) + tot[0]);
+        const double pot = tot[1] - P.m0 * gs[G_G1];
+        const double pn = P.lam * tot[2], pn2 = P.lam * tot[3];
+        out[0] = kin;
+        out[1] = pot;
+        out[2] = pn;
+        out[3] = pn2;
+        out[4] = kin + pot + pn + pn2;
+    }
+}
+
+}  // namespace pn1dev

@@ -1,0 +1,142 @@
+"""HIP path (libpn1 through its C ABI) against the reference's golden vectors
+and the oracle.
+
+Tolerances:
+  FP64 form, accelerations: per-particle ||a - a_ref|| / ||a_ref|| <= 1e-12.
+    One exception is written out below: the reference's a1c bracket
+    (Limit-sticky9.c:139-140) computes 0.25 D2_j/(r^2+eps^2) w and
+    0.25 D2_i/(r^2+eps^2) w separately and subtracts them; for a pair much
+    closer than its distance to the BH both terms are ~D2/r^3 and the
+    difference loses digits in the reference itself.  The BH's acceleration
+    is then compared with an absolute bound of a few ulp of those terms.
+  FP64 form, energies: energy_atol (self-pair rounding of the reference).
+  Mixed form: against the FP64 form, per-particle relative error
+    p99.9 <= 1e-6 and max <= 1e-5 (stars), BH <= 1e-6, for c >= 50.  At
+    c = 1 (the reference's own units) the 1PN three-body terms, which the
+    mixed form evaluates in FP32, are as large as the Newtonian ones, and
+    the bound is p99.9 <= 3e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import FORCE_FIXTURES, c_values, energy_atol, load_fixture, rel_err
+from pn_amd import lib as pn
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pn.Context(capacity=20000)
+    yield c
+    c.close()
+
+
+def gpu_force(ctx, d, c, mode=pn.FP64):
+    ctx.set_params(float(d["eps"]), float(c), 2.0**-10, mode)
+    ctx.upload(d["x"], d["v"], d["m"])
+    return ctx.force()
+
+
+def bh_a1c_noise(d, c):
+    """Scale of the reference's own rounding in the BH's a1c pair sum:
+    sum_ij m_i e_i^3 m_j |d_i| 0.25 max(D2_i, D2_j) w^3 times ulp, / c^2."""
+    x, m, eps = d["x"], d["m"], float(d["eps"])
+    dd = x[1:] - x[0]
+    D2 = (dd * dd).sum(1)
+    e = 1.0 / np.sqrt(D2 + eps * eps)
+    r = x[1:][None, :, :] - x[1:][:, None, :]
+    w = 1.0 / np.sqrt((r * r).sum(2) + eps * eps)
+    big = 0.25 * np.maximum(D2[:, None], D2[None, :]) * w**3
+    t = (m[1:] * e**3 * np.sqrt(D2))[:, None] * m[1:][None, :] * big
+    return 8 * 1.1e-16 * t.sum() / (c * c)
+
+
+# Per-case star tolerance where the reference is itself less accurate than
+# 1e-12.  close_n65 at c = 1 puts a star 2e-4 from the BH (m0/r = 250 c^2):
+# its three-body 1PN terms cancel ~2000-fold, and the reference differs from
+# an 80-bit evaluation of the same formulas by 3.4e-11 (stars).
+STAR_TOL = {("close_n65", 1): 1e-10}
+
+
+@pytest.mark.parametrize("name", FORCE_FIXTURES)
+def test_fp64_force_vs_reference(ctx, name):
+    d = load_fixture(name)
+    for c in [1] + c_values(d):
+        a = gpu_force(ctx, d, c)
+        ref = d[f"a_c{c}"]
+        err = rel_err(a, ref)
+        print(f"{name} c={c}: star max rel {err[1:].max() if len(err) > 1 else 0:.3e}  "
+              f"BH rel {err[0]:.3e}")
+        if len(err) > 1:
+            assert err[1:].max() <= STAR_TOL.get((name, c), 1e-12), (name, c, err[1:].max())
+        bh_abs = np.linalg.norm(a[0] - ref[0])
+        assert err[0] <= 1e-12 or bh_abs <= bh_a1c_noise(d, c), (name, c, err[0], bh_abs)
+
+
+@pytest.mark.parametrize("name", FORCE_FIXTURES)
+def test_fp64_energy_vs_reference(ctx, name):
+    d = load_fixture(name)
+    ctx.set_params(float(d["eps"]), 1.0, 2.0**-10, pn.FP64)
+    ctx.upload(d["x"], d["v"], d["m"])
+    e = ctx.energy()
+    E0 = float(d["E0_c1"])
+    assert abs(e[4] - E0) <= energy_atol(d, np.array([E0])), (name, e[4], E0)
+    for c in c_values(d):
+        ctx.set_params(float(d["eps"]), float(c), 2.0**-10, pn.FP64)
+        e = ctx.energy()
+        ref = d[f"E_c{c}"]
+        mine = np.array([e[0], e[1], e[2], e[4]])
+        assert np.all(np.abs(mine - ref) <= energy_atol(d, ref)), (name, c, mine, ref)
+
+
+def test_fp64_force_vs_oracle_larger(ctx):
+    """n = 4097 seeded sphere (oracle: ~1 s)."""
+    x, v, m = pn.ic_uniform_sphere(4096, 7, R=1.0, Q=0.5, m_bh=0.01, eps=1e-3)
+    for c in (1.0, 100.0):
+        ctx.set_params(1e-3, c, 2.0**-10, pn.FP64)
+        ctx.upload(x, v, m)
+        a = ctx.force()
+        ref = ob.orc_force(x, v, m, 1e-3, c)
+        err = rel_err(a, ref)
+        print(f"n=4097 c={c}: star max {err[1:].max():.3e} BH {err[0]:.3e}")
+        assert err[1:].max() <= 1e-12
+        assert err[0] <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["traj_rand_n65_c1", "traj_c1_n1025_c100"])
+def test_fp64_trajectory(ctx, name):
+    d = load_fixture(name)
+    ctx.set_params(float(d["eps"]), float(d["c"]), float(d["dt"]), pn.FP64)
+    ctx.upload(d["x"], d["v"], d["m"])
+    ctx.step(int(d["steps"]))
+    x, v = ctx.download()
+    ex = np.abs(x - d["xK"]).max() / np.abs(d["xK"]).max()
+    ev = np.abs(v - d["vK"]).max() / np.abs(d["vK"]).max()
+    print(f"{name}: state rel err x {ex:.3e} v {ev:.3e}")
+    assert ex <= 1e-12 and ev <= 1e-12
+
+
+def test_legacy_shapes(ctx):
+    d = load_fixture("c1_n1025")
+    pn.set_c(1.0)
+    pn.set_mode(pn.FP64)
+    a = pn.legacy_force(d["x"], d["v"], d["m"], float(d["eps"]))
+    err = rel_err(a, d["a_c1"])
+    assert err.max() <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["rand_n257", "c1_n1025", "dat_n65"])
+def test_mixed_vs_fp64(ctx, name):
+    d = load_fixture(name)
+    for c in [1] + c_values(d):
+        a64 = gpu_force(ctx, d, c, pn.FP64)
+        a32 = gpu_force(ctx, d, c, pn.MIXED)
+        err = rel_err(a32, a64)
+        stars = err[1:]
+        print(f"{name} c={c}: mixed star p99.9 {np.quantile(stars, 0.999):.3e} "
+              f"max {stars.max():.3e} BH {err[0]:.3e}")
+        assert np.quantile(stars, 0.999) <= (3e-6 if c == 1 else 1e-6)
+        assert stars.max() <= 1e-5
+        assert err[0] <= 1e-6

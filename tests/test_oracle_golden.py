@@ -1,0 +1,59 @@
+"""The oracle against the golden vectors written from the compiled reference."""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import c_values, energy_atol, load_fixture, rel_err
+
+
+def test_force_c1_bitwise(force_fixture):
+    d = force_fixture
+    a = ob.orc_force(d["x"], d["v"], d["m"], float(d["eps"]), 1.0)
+    assert np.array_equal(a.view(np.uint64), d["a_c1"].view(np.uint64)), d["name"]
+
+
+def test_initial_energy_c1_bitwise_and_text(force_fixture):
+    d = force_fixture
+    E0, txt = ob.orc_initial_energy_text(d["x"], d["v"], d["m"], float(d["eps"]), 1.0)
+    assert E0 == float(d["E0_c1"])
+    assert txt == str(d["init_txt_c1"])
+
+
+def test_energy_text_c1(force_fixture):
+    d = force_fixture
+    txt = ob.orc_energy_text(0.25, d["x"], d["v"], d["m"], float(d["E0_c1"]) * (1.0 + 1e-6),
+                             float(d["eps"]), 1.0)
+    assert txt == str(d["energy_txt_c1"])
+
+
+def test_force_speed_of_light(force_fixture):
+    """At c != 1 the reference is run on v/c, m/c^2 and rescaled; that adds a
+    few roundings per term, so agreement is to ~1e-13 rather than bitwise."""
+    d = force_fixture
+    for c in c_values(d):
+        a = ob.orc_force(d["x"], d["v"], d["m"], float(d["eps"]), float(c))
+        err = rel_err(a, d[f"a_c{c}"])
+        assert err.max() < 1e-12, (d["name"], c, err.max())
+
+
+def test_energy_speed_of_light(force_fixture):
+    d = force_fixture
+    for c in c_values(d):
+        e = ob.orc_energy_terms(d["x"], d["v"], d["m"], float(d["eps"]), float(c))
+        ref = d[f"E_c{c}"]           # kin, pot, pot_pn, total
+        mine = np.array([e[0], e[1], e[2], e[4]])
+        assert np.all(np.abs(mine - ref) <= energy_atol(d, ref)), (d["name"], c)
+
+
+@pytest.mark.parametrize("name,tol", [("traj_rand_n65_c1", 0.0), ("traj_c1_n1025_c100", 1e-12)])
+def test_q1_trajectory(name, tol):
+    """The oracle's stage driver against a stage loop around the reference's
+    force(): bitwise at c = 1 (same force, same stage arithmetic)."""
+    d = load_fixture(name)
+    x, v = ob.orc_q1_steps(d["x"], d["v"], d["m"], float(d["eps"]), float(d["c"]),
+                           float(d["dt"]), int(d["steps"]))
+    if tol == 0.0:
+        assert np.array_equal(x, d["xK"]) and np.array_equal(v, d["vK"])
+    else:
+        assert np.abs(x - d["xK"]).max() <= tol * np.abs(d["xK"]).max()
+        assert np.abs(v - d["vK"]).max() <= tol * np.abs(d["vK"]).max()
