@@ -1,0 +1,199 @@
+"""Headline benchmark: interactions/s of the force path at N = 131072.
+
+Workload (one MI355X): a seeded Plummer sphere of 131072 stars (scale radius
+1, total star mass 1) plus a central black hole of mass 0.05, speed of light
+c = 50, Plummer softening eps = 1e-4, RK step dt = 2^-12.  A step is one step
+of the reference's stage driver (Limit-sticky9.c:841-893): three force
+evaluations of n^2 interactions each (n = 131073 counts the black hole), plus
+the O(n) closed forms and the stage update, with the state resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints ONE JSON line.  `value` is the mixed-precision form's interactions/s
+(star-star pairs in FP32 on double-single positions, everything else FP64);
+the FP64 form is measured the same way and reported beside it, with the
+per-particle acceleration error of the mixed form against the FP64 form at
+the same state.  The pair kernel's average launch duration comes from HIP
+events recorded around every pair launch on the library's stream during the
+timed steps.  Algorithmic work: 90 flops per interaction (FMA = 2,
+rsqrt = 1; Newtonian 20 + three-body 1PN 70).
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from pn_amd import lib as pn  # noqa: E402
+
+METRIC = ("interactions/s & % vector peak at N=128k, mixed vs FP64 GPU @ acc rel err"
+          "≤1e-6")
+N_STARS, SEED, M_BH, C_LIGHT, EPS, DT = 131072, 3, 0.05, 50.0, 1e-4, 2.0**-12
+FLOPS_PER_INTERACTION = 90
+PEAK_FP32 = 157.3e12   # MI355X vector FP32
+PEAK_FP64 = 78.6e12    # MI355X vector FP64
+CPU_SAMPLE_STARS = 16384
+
+
+def rel_err(a, ref):
+    return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+
+
+def run_form(ctx, x, v, m, mode, steps, warmup):
+    ctx.set_params(EPS, C_LIGHT, DT, mode)
+    ctx.upload(x, v, m)
+    e0 = ctx.energy()
+    ctx.profile_enable(True)
+    if warmup:
+        ctx.step(warmup)
+    ctx.sync()
+    ctx.profile_enable(True)          # drop warm-up launches from the kernel timing
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    launches, pair_ms = ctx.profile_read()
+    ctx.profile_enable(False)
+    e1 = ctx.energy()
+    n = len(m)
+    wall = t1 - t0
+    return dict(
+        interactions_per_s=3.0 * n * n * steps / wall,
+        ms_per_step=1e3 * wall / steps,
+        pair_launches=launches,
+        pair_avg_ms=pair_ms / max(launches, 1),
+        energy_drift=float((e1[4] - e0[4]) / abs(e0[4])),
+    )
+
+
+def roofline(n, avg_ms, peak, traffic=None):
+    flops = FLOPS_PER_INTERACTION * float(n) * n
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak / 1e12,
+            "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4), "traffic": traffic}
+
+
+def load_traffic(kind):
+    """HBM bytes per pair launch from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "pair_kernel_traffic_n131073.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kind)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(x, v, m):
+    """The reference's own force() (compiled from its sources into
+    oracle/_ref) on the first CPU_SAMPLE_STARS stars of the workload, one
+    call on one core; the oracle restatement if that library is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    n = CPU_SAMPLE_STARS + 1
+    xs, vs, ms = x[:n].copy(), v[:n].copy(), m[:n].copy()
+    # the reference has no speed of light: run it on v/c, m/c^2 (same work)
+    vr, mr = vs / C_LIGHT, ms / C_LIGHT**2
+    if ob.have_ref():
+        kind, fn = "reference", lambda: ob.ref_force(xs, vr, mr, EPS)
+    else:
+        kind, fn = "port", lambda: ob.orc_force(xs, vr, mr, EPS, 1.0)
+    t0 = time.perf_counter()
+    fn()
+    dt = time.perf_counter() - t0
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": float(n) * n / dt, "unit": "interactions/s", "cores": 1, "kind": kind,
+            "sample": f"one force() call, first {CPU_SAMPLE_STARS} stars + BH of the workload "
+                      f"(n={n}, {n * n:.3e} interactions, {dt:.1f} s, gcc -O2, 1 thread, {cpu})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--jsplit", type=int, default=0)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this path runs on one GPU; there is no multi-GPU mode "
+              f"(asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+
+    x, v, m = pn.ic_plummer(N_STARS, SEED, a=1.0, m_bh=M_BH)
+    n = len(m)
+    with pn.Context(capacity=n, device=0) as ctx:
+        ctx.set_tuning(args.jsplit, args.tile, args.variant)
+        # accuracy of the mixed form at this state (one evaluation each)
+        ctx.set_params(EPS, C_LIGHT, DT, pn.FP64)
+        ctx.upload(x, v, m)
+        a64 = ctx.force()
+        ctx.set_params(EPS, C_LIGHT, DT, pn.MIXED)
+        ctx.upload(x, v, m)
+        a32 = ctx.force()
+        err = rel_err(a32, a64)
+        fp64 = run_form(ctx, x, v, m, pn.FP64, args.steps, args.warmup)
+        mixed = run_form(ctx, x, v, m, pn.MIXED, args.steps, args.warmup)
+        jsplit, tile, variant = ctx.get_tuning()
+
+    out = {
+        "metric": METRIC,
+        "value": mixed["interactions_per_s"],
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pairs on double-single f32 positions; f64 BH-star 1PN, RK update, energy",
+        "data": "synthetic: seeded Plummer sphere (AHW 1974 sampling, seed 3), FP32-exact",
+        "config": {"workload": "Plummer N=131072 + BH m=0.05, c=50, eps=1e-4, dt=2^-12, "
+                               "3 force evaluations per step",
+                   "n": n, "interactions_per_step": 3 * n * n, "jsplit": jsplit, "tile": tile,
+                   "variant": variant},
+        "roofline": roofline(n, mixed["pair_avg_ms"], PEAK_FP32, load_traffic("mixed")),
+        "fp64": {"interactions_per_s": fp64["interactions_per_s"],
+                 "ms_per_step": fp64["ms_per_step"],
+                 "roofline": roofline(n, fp64["pair_avg_ms"], PEAK_FP64, load_traffic("fp64")),
+                 "energy_drift": fp64["energy_drift"]},
+        "mixed_over_fp64": mixed["interactions_per_s"] / fp64["interactions_per_s"],
+        "mixed_accuracy_vs_fp64": {"star_rel_err_max": float(err[1:].max()),
+                                   "star_rel_err_p999": float(np.quantile(err[1:], 0.999)),
+                                   "bh_rel_err": float(err[0]),
+                                   "energy_drift_mixed": mixed["energy_drift"]},
+        "pair_kernel_avg_ms": {"mixed": mixed["pair_avg_ms"], "fp64": fp64["pair_avg_ms"],
+                               "launches": mixed["pair_launches"]},
+        "flops_per_interaction": FLOPS_PER_INTERACTION,
+    }
+    if not args.no_cpu:
+        old = threading.stack_size(256 << 20)
+        res = {}
+        t = threading.Thread(target=lambda: res.setdefault("r", cpu_baseline(x, v, m)))
+        t.start()
+        t.join()
+        threading.stack_size(old)
+        out["cpu_baseline"] = res.get("r")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -103,10 +103,11 @@ int  pn1_sync(pn1_ctx *ctx);
 const char *pn1_last_error(const pn1_ctx *ctx);
 
 /* Launch shape of the pair kernels: the j range is split into `jsplit`
- * chunks (0 = automatic), and the mixed kernel compensates its FP32 sums
- * every `tile` pairs (0 = default).  Used by the tuner. */
-int  pn1_set_tuning(pn1_ctx *ctx, int jsplit, int tile);
-int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile);
+ * chunks (0 = automatic), the mixed kernel compensates its FP32 sums every
+ * `tile` pairs (0 = default), and `variant` picks the j-loop form (0: one
+ * j-record at a time, 1: two in flight).  Used by the tuner. */
+int  pn1_set_tuning(pn1_ctx *ctx, int jsplit, int tile, int variant);
+int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile, int *variant);
 
 /* Timing of the pair kernel: while enabled, pn1_step/pn1_force launch
  * without a graph and bracket every force pair-kernel launch with HIP
@@ -114,6 +115,11 @@ int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile);
  * of pair launches and their summed duration (ms) since enabling. */
 int  pn1_profile_enable(pn1_ctx *ctx, int on);
 int  pn1_profile_read(pn1_ctx *ctx, int *launches, double *total_ms);
+
+/* Copies `bytes` between two scratch device buffers with a plain copy
+ * kernel, `reps` times (allocates and frees them): a workload of known
+ * HBM traffic for calibrating profiler byte counters. */
+int  pn1_util_copy(pn1_ctx *ctx, long long bytes, int reps);
 
 /* ====================================================================== */
 /* 3. Host-side helpers (no device needed).                               */

@@ -33,9 +33,9 @@ struct pn1_ctx {
     double eps = 1e-3, c = 1.0, dt = 1.0 / 1024.0;
     int mode = PN1_FP64;
     double m0 = 0.0;            // host copy of m[0]
-    int jsplit_req = 0, tile_req = 0;
+    int jsplit_req = 0, tile_req = 0, variant = 0;
     // launch plan for the current n
-    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 32;
+    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 64;
     // device buffers
     double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr;
     double *m = nullptr, *acc = nullptr, *gblk = nullptr, *gs = nullptr;
@@ -123,7 +123,7 @@ void plan(pn1_ctx* c)
     } else {
         c->pair_grid = S * c->nIblk;
     }
-    c->tile = c->tile_req > 0 ? c->tile_req : 32;
+    c->tile = c->tile_req > 0 ? c->tile_req : 64;
 }
 
 int ensure_part(pn1_ctx* c)
@@ -135,7 +135,7 @@ int ensure_part(pn1_ctx* c)
     if (needE > need) need = needE;
     if (need32 > need) need = need32;
     if (need <= c->part_bytes) return PN1_OK;
-    if (c->part) hipFree(c->part);
+    if (c->part) (void)hipFree(c->part);
     c->part = nullptr;
     c->part_bytes = 0;
     if (hipMalloc(&c->part, need) != hipSuccess) {
@@ -148,7 +148,7 @@ int ensure_part(pn1_ctx* c)
 
 void drop_graph(pn1_ctx* c)
 {
-    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
 }
 
@@ -194,12 +194,22 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
             HIPCHK(c, hipEventCreate(&e1));
             HIPCHK(c, hipEventRecord(e0, c->st));
         }
-        if (mixed)
-            hipLaunchKernelGGL(k_pair32, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j32,
-                               (float*)c->part, P, c->S, c->nIblk, c->tile);
-        else
-            hipLaunchKernelGGL(k_pair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64,
-                               (double*)c->part, P, c->S, c->nIblk);
+        const dim3 grid(c->pair_grid), blk(WG);
+        if (mixed) {
+            if (c->variant)
+                hipLaunchKernelGGL(k_pair32<1>, grid, blk, 0, c->st, c->j32, in,
+                                   (float*)c->part, P, c->S, c->nIblk, c->tile);
+            else
+                hipLaunchKernelGGL(k_pair32<0>, grid, blk, 0, c->st, c->j32, in,
+                                   (float*)c->part, P, c->S, c->nIblk, c->tile);
+        } else {
+            if (c->variant)
+                hipLaunchKernelGGL(k_pair64<1>, grid, blk, 0, c->st, c->j64, in,
+                                   (double*)c->part, P, c->S, c->nIblk);
+            else
+                hipLaunchKernelGGL(k_pair64<0>, grid, blk, 0, c->st, c->j64, in,
+                                   (double*)c->part, P, c->S, c->nIblk);
+        }
         if (c->prof) {
             HIPCHK(c, hipEventRecord(e1, c->st));
             c->ev.emplace_back(e0, e1);
@@ -253,22 +263,22 @@ const char* pn1_last_error(const pn1_ctx* c) { return c ? c->err : "null context
 void pn1_destroy(pn1_ctx* c)
 {
     if (!c) return;
-    if (c->dev >= 0) hipSetDevice(c->dev);
-    if (c->st) hipStreamSynchronize(c->st);
+    if (c->dev >= 0) (void)hipSetDevice(c->dev);
+    if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graph(c);
     for (auto& e : c->ev) {
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
     double* bufs[] = {c->X, c->Y, c->H1, c->H2, c->m, c->acc, c->gblk, c->gs, c->bhblk, c->eblk,
                       c->eout};
     for (double* b : bufs)
-        if (b) hipFree(b);
-    if (c->j64) hipFree(c->j64);
-    if (c->j32) hipFree(c->j32);
-    if (c->part) hipFree(c->part);
-    if (c->h_stage) hipHostFree(c->h_stage);
-    if (c->st) hipStreamDestroy(c->st);
+        if (b) (void)hipFree(b);
+    if (c->j64) (void)hipFree(c->j64);
+    if (c->j32) (void)hipFree(c->j32);
+    if (c->part) (void)hipFree(c->part);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
 
@@ -309,8 +319,9 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
     if (!rc) rc = dalloc(c, &c->bhblk, nb * 3);
     if (!rc) rc = dalloc(c, &c->eblk, nb * NEB);
     if (!rc) rc = dalloc(c, &c->eout, 8);
-    if (!rc) rc = dalloc(c, &c->j64, cap);
-    if (!rc) rc = dalloc(c, &c->j32, cap);
+    // two records more than particles: the pair loops fetch up to jr[j + 2]
+    if (!rc) rc = dalloc(c, &c->j64, cap + 2);
+    if (!rc) rc = dalloc(c, &c->j32, cap + 2);
     if (!rc && hipHostMalloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
         c->h_stage = nullptr;
         rc = PN1_ENOMEM;
@@ -338,13 +349,15 @@ int pn1_set_params(pn1_ctx* c, double eps, double cl, double dt, int mode)
     return PN1_OK;
 }
 
-int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile)
+int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile, int variant)
 {
     if (!c) return PN1_EINVAL;
-    if (jsplit < 0 || jsplit > 4096 || tile < 0 || tile > 65536)
-        return seterr(c, PN1_EINVAL, "bad tuning (jsplit %d, tile %d)", jsplit, tile);
+    if (jsplit < 0 || jsplit > 4096 || tile < 0 || tile > 65536 || variant < 0 || variant > 1)
+        return seterr(c, PN1_EINVAL, "bad tuning (jsplit %d, tile %d, variant %d)", jsplit, tile,
+                      variant);
     c->jsplit_req = jsplit;
     c->tile_req = tile;
+    c->variant = variant;
     drop_graph(c);
     if (!c->have_state) return PN1_OK;
     HIPCHK(c, hipSetDevice(c->dev));
@@ -353,11 +366,12 @@ int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile)
     return ensure_part(c);
 }
 
-int pn1_get_tuning(const pn1_ctx* c, int* jsplit, int* tile)
+int pn1_get_tuning(const pn1_ctx* c, int* jsplit, int* tile, int* variant)
 {
     if (!c) return PN1_EINVAL;
     if (jsplit) *jsplit = c->S;
     if (tile) *tile = c->tile;
+    if (variant) *variant = c->variant;
     return PN1_OK;
 }
 
@@ -421,7 +435,7 @@ int pn1_energy(pn1_ctx* c, pn1_energy_t* out)
                        (JRec32*)nullptr, c->gblk);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     if (c->n > 1)
-        hipLaunchKernelGGL(k_epair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64,
+        hipLaunchKernelGGL(k_epair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64, c->X,
                            (double*)c->part, P, c->S, c->nIblk);
     hipLaunchKernelGGL(k_efin, dim3(c->nblk), dim3(WG), 0, c->st, c->X, c->m,
                        (const double*)c->part, c->n > 1 ? c->S : 0, c->gs, c->eblk, P);
@@ -457,13 +471,13 @@ int pn1_step(pn1_ctx* c, int steps)
         rc = enqueue_step(c, P);
         hipError_t ec = hipStreamEndCapture(c->st, &g);
         if (rc) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         if (ec != hipSuccess)
             return seterr(c, PN1_EDEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
         hipError_t ei = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
             c->gexec = nullptr;
             return seterr(c, PN1_EDEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
@@ -487,8 +501,8 @@ int pn1_profile_enable(pn1_ctx* c, int on)
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->st));
     for (auto& e : c->ev) {
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
     c->ev.clear();
     c->prof = on != 0;
@@ -507,12 +521,36 @@ int pn1_profile_read(pn1_ctx* c, int* launches, double* total_ms)
         HIPCHK(c, hipEventElapsedTime(&ms, e.first, e.second));
         c->prof_ms += ms;
         c->prof_launches += 1;
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
     c->ev.clear();
     if (launches) *launches = c->prof_launches;
     if (total_ms) *total_ms = c->prof_ms;
+    return PN1_OK;
+}
+
+// Copies `bytes` (rounded down to 16) between two device buffers with
+// k_copy on the context's stream, `reps` times; for counter calibration.
+int pn1_util_copy(pn1_ctx* c, long long bytes, int reps)
+{
+    if (!c || bytes < 16 || reps < 1) return PN1_EINVAL;
+    HIPCHK(c, hipSetDevice(c->dev));
+    const size_t n2 = (size_t)bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n2 * 16) != hipSuccess) return seterr(c, PN1_ENOMEM, "copy buffer");
+    if (hipMalloc(&b, n2 * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return seterr(c, PN1_ENOMEM, "copy buffer");
+    }
+    (void)hipMemsetAsync(a, 0, n2 * 16, c->st);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_copy, dim3(2048), dim3(WG), 0, c->st, (const double2*)a, (double2*)b,
+                           n2);
+    hipError_t e = hipStreamSynchronize(c->st);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return seterr(c, PN1_EDEVICE, "copy: %s", hipGetErrorString(e));
     return PN1_OK;
 }
 

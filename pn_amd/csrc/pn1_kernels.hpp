@@ -28,39 +28,52 @@
 namespace pn1dev {
 
 constexpr int WG = 256;     // threads per workgroup (4 waves of 64)
-constexpr int NACC = 15;    // force pair accumulators
+constexpr int NACC = 16;    // force pair accumulators
 constexpr int NACCE = 5;    // energy pair accumulators
 constexpr int NGS = 7;      // separable O(n) sums
 constexpr int NEB = 4;      // energy per-block partials
 
-// force pair accumulators (per star i, sums over stars j != i; t = |r|^2 - D2_i):
+// sqrt(3/2): stored velocities are pre-scaled by it, so that the pair term
+// 1.5 (v_j.r)^2 w^2 is a plain square of (s_j.r) w with s_j = sqrt(3/2) v_j.
+#define PN1_SQ15 1.2247448713915890491
+// 4 / sqrt(3/2) and 3 / sqrt(3/2) = sqrt(6)
+#define PN1_4OSQ15 3.2659863237109041
+#define PN1_SQ6 2.4494897427831781
+
+// force pair accumulators (per star i, sums over stars j != i; t = |r|^2 - D2_i,
+// s_j = sqrt(3/2) v_j, K_u = m_j w^3 (4 v_i.r - 3 v_j.r)):
 //   N    = sum m_j w^3 r                                      (Newton)
-//   Rr   = sum K_r r   K_r = m_j w^3 [m0(4e_i + 1.25e_j + 0.25e_j^3 t)
-//                                       + |v_i|^2 - 2|u|^2 + 1.5 (v_j.r)^2 w^2]
-//   Ru   = sum m_j w^3 (r.(4v_i - 3v_j)) u
-//   T3   = sum 3.5 m0 m_j e_j^3 (w - e_i) d_j
+//   Rr   = sum K_r r,  K_r = m_j w^3 [m0(4e_i + 1.25e_j + 0.25e_j^3 t)
+//                                      + |v_i|^2 - 2|u|^2 + 1.5 (v_j.r)^2 w^2]
+//          using |v_i|^2 - 2|u|^2 = -|v_i|^2 - 2|v_j|^2 + 4 v_i.v_j
+//   U    = sum K_u s_j,  KS = sum K_u      (sum K_u u = U / sqrt(3/2) - v_i KS)
+//   T3   = sum 3.5 m0 m_j e_j^3 w d_j      (k_fin subtracts e_i times its w-free sum)
 //   phi  = sum m_j w,  xi = sum m_j w^3 (D2_i - D2_j),  zeta = sum m_j 0.25 m0 e_j^3 t
-// T3 keeps the reference's grouping (w - e_i) d_j (Limit-sticky9.c:307-308),
-// and zeta shares t with K_r, as the reference's two e_j^3 terms share
-// r1b2 - rbc2 (:302, 305): both pairs of terms cancel strongly when star j
-// is close to the BH, and splitting them differently loses digits.
+// zeta shares t with K_r, as the reference's two e_j^3 terms share
+// r1b2 - rbc2 (Limit-sticky9.c:302, 305); the two cancel strongly when star
+// j is close to the BH, and the shared rounding cancels with them.  The
+// (w - e_i) c_j term (:307-308) is summed over d_j = x_j - x_BH, never over
+// r + d_i, which would cancel catastrophically in the same situation.
 enum { A_NX, A_NY, A_NZ, A_RX, A_RY, A_RZ, A_UX, A_UY, A_UZ, A_TX, A_TY, A_TZ,
-       A_PHI, A_XI, A_ZE };
-// energy pair accumulators: phi, sum m_j w v_j, sum m_j w^3 (r.v_i)(r.v_j)
+       A_KS, A_PHI, A_XI, A_ZE };
+// energy pair accumulators: phi, sum m_j w s_j, sum m_j w^3 (r.v_i)(r.s_j)
 enum { E_PHI, E_PVX, E_PVY, E_PVZ, E_GAM };
 // separable sums over stars: sum m e, sum m e^3 d, sum m v
 enum { G_G1, G_M1X, G_M1Y, G_M1Z, G_PX, G_PY, G_PZ };
 
-// j-record, FP64 form: position, velocity, mass, |d|^2, 1.25 m0 e,
-// 0.25 m0 e^3, 3.5 m0 m e^3, e, d = x - x_BH, m 0.25 m0 e^3.  128 bytes =
-// two 16-dword scalar loads.
+// j-record, FP64 form, 128 bytes (two 16-dword scalar loads):
+//   x            position                 s = sqrt(3/2) v
+//   m, D2        mass, |d|^2, d = x - x_BH
+//   A            1.25 m0 e - 2 |v|^2       B = 0.25 m0 e^3
+//   Q            3.5 m0 m e^3              MB = m B
+//   d            x - x_BH                  e = (|d|^2 + eps^2)^-1/2
 struct __align__(16) JRec64 {
-    double x, y, z, vx, vy, vz, m, D2, A, B, Q, e, dx, dy, dz, MB;
+    double x, y, z, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e;
 };
 // j-record, mixed form: double-single position (hi, lo), the rest rounded
 // to FP32.  80 bytes.
 struct __align__(16) JRec32 {
-    float xh, yh, zh, xl, yl, zl, vx, vy, vz, m, D2, A, B, Q, e, dx, dy, dz, MB, pad;
+    float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e, pad;
 };
 static_assert(sizeof(JRec64) == 128, "JRec64 layout");
 static_assert(sizeof(JRec32) == 80, "JRec32 layout");
@@ -188,11 +201,14 @@ __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
         const double e = 1.0 / sqrt(D2 + P.eps2);     // as Limit-sticky9.c:106-108
         const double E3 = e * e * e;
         const double q = mp * E3;
+        const double V2 = vx * vx + vy * vy + vz * vz;
         JRec64 r;
-        r.x = x; r.y = y; r.z = z; r.vx = vx; r.vy = vy; r.vz = vz;
+        r.x = x; r.y = y; r.z = z;
+        r.sx = PN1_SQ15 * vx; r.sy = PN1_SQ15 * vy; r.sz = PN1_SQ15 * vz;
         r.m = mp; r.D2 = D2;
-        r.A = 1.25 * P.m0 * e; r.B = 0.25 * P.m0 * E3; r.Q = 3.5 * P.m0 * q; r.e = e;
-        r.dx = dx; r.dy = dy; r.dz = dz; r.MB = mp * r.B;
+        r.A = 1.25 * P.m0 * e - 2.0 * V2; r.B = 0.25 * P.m0 * E3;
+        r.Q = 3.5 * P.m0 * q; r.MB = mp * r.B;
+        r.dx = dx; r.dy = dy; r.dz = dz; r.e = e;
         j64[p] = r;
         if (j32) {
             JRec32 h;
@@ -200,10 +216,10 @@ __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
             h.xl = (float)(x - (double)h.xh);
             h.yl = (float)(y - (double)h.yh);
             h.zl = (float)(z - (double)h.zh);
-            h.vx = (float)vx; h.vy = (float)vy; h.vz = (float)vz;
+            h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
             h.m = (float)mp; h.D2 = (float)D2;
-            h.A = (float)r.A; h.B = (float)r.B; h.Q = (float)r.Q; h.e = (float)e;
-            h.dx = (float)dx; h.dy = (float)dy; h.dz = (float)dz; h.MB = (float)r.MB;
+            h.A = (float)r.A; h.B = (float)r.B; h.Q = (float)r.Q; h.MB = (float)r.MB;
+            h.dx = (float)dx; h.dy = (float)dy; h.dz = (float)dz; h.e = (float)e;
             h.pad = 0.0f;
             j32[p] = h;
         }
@@ -231,23 +247,25 @@ __global__ __launch_bounds__(WG) void k_rows_sum(const double* __restrict__ rows
 // ---------------------------------------------------------------------
 // force pair kernel, FP64
 // ---------------------------------------------------------------------
+// Per-lane state of the i particle: position, g = 4 v, h = 4 v / sqrt(3/2),
+// al = 4 m0 e - |v|^2, |d|^2 and |d|^2 + eps^2.
 struct I64 {
-    double x, y, z, vx, vy, vz, v4x, v4y, v4z, al, D2, D2e, e;
+    double x, y, z, gx, gy, gz, hx, hy, hz, al, D2, D2e;
 };
 
-__device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr, int i, double m0,
-                                        double eps2)
+__device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr,
+                                        const double* __restrict__ st, int cap, int i,
+                                        double m0, double eps2)
 {
-    const JRec64 r = jr[i];
+    const JRec64& r = jr[i];
+    const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
     I64 I;
     I.x = r.x; I.y = r.y; I.z = r.z;
-    I.vx = r.vx; I.vy = r.vy; I.vz = r.vz;
-    I.v4x = 4.0 * r.vx; I.v4y = 4.0 * r.vy; I.v4z = 4.0 * r.vz;
-    const double V2 = r.vx * r.vx + r.vy * r.vy + r.vz * r.vz;
-    I.al = 4.0 * m0 * r.e + V2;
+    I.gx = 4.0 * vx; I.gy = 4.0 * vy; I.gz = 4.0 * vz;
+    I.hx = PN1_4OSQ15 * vx; I.hy = PN1_4OSQ15 * vy; I.hz = PN1_4OSQ15 * vz;
+    I.al = 4.0 * m0 * r.e - (vx * vx + vy * vy + vz * vz);
     I.D2 = r.D2;
     I.D2e = r.D2 + eps2;
-    I.e = r.e;
     return I;
 }
 
@@ -255,11 +273,12 @@ template <bool MASK>
 __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64& I,
                                        double (&a)[NACC], double eps2)
 {
-    double mj = J.m, Qj = J.Q;
+    double mj = J.m, Qj = J.Q, MBj = J.MB;
     if (MASK) {
         const bool self = (j == i);
         mj = self ? 0.0 : mj;
         Qj = self ? 0.0 : Qj;
+        MBj = self ? 0.0 : MBj;
     }
     const double rx = J.x - I.x, ry = J.y - I.y, rz = J.z - I.z;
     const double R2e = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
@@ -272,28 +291,50 @@ __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64&
     a[A_NZ] = __builtin_fma(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
     a[A_XI] = __builtin_fma(mw3, I.D2 - J.D2, a[A_XI]);
-    const double c3 = Qj * (w - I.e);
-    a[A_TX] = __builtin_fma(c3, J.dx, a[A_TX]);
-    a[A_TY] = __builtin_fma(c3, J.dy, a[A_TY]);
-    a[A_TZ] = __builtin_fma(c3, J.dz, a[A_TZ]);
+    const double qw = Qj * w;
+    a[A_TX] = __builtin_fma(qw, J.dx, a[A_TX]);
+    a[A_TY] = __builtin_fma(qw, J.dy, a[A_TY]);
+    a[A_TZ] = __builtin_fma(qw, J.dz, a[A_TZ]);
     const double t = R2e - I.D2e;
-    a[A_ZE] = __builtin_fma(mj * J.B, t, a[A_ZE]);
-    const double ux = J.vx - I.vx, uy = J.vy - I.vy, uz = J.vz - I.vz;
-    const double uu = __builtin_fma(ux, ux, __builtin_fma(uy, uy, uz * uz));
-    const double vjr = __builtin_fma(J.vx, rx, __builtin_fma(J.vy, ry, J.vz * rz));
+    a[A_ZE] = __builtin_fma(MBj, t, a[A_ZE]);
+    const double sr = __builtin_fma(J.sx, rx, __builtin_fma(J.sy, ry, J.sz * rz));
     double br = __builtin_fma(J.B, t, I.al + J.A);
-    br = __builtin_fma(-2.0, uu, br);
-    const double tv = vjr * w;
-    br = __builtin_fma(1.5 * tv, tv, br);
+    br = __builtin_fma(I.hx, J.sx, __builtin_fma(I.hy, J.sy, __builtin_fma(I.hz, J.sz, br)));
+    const double tv = sr * w;
+    br = __builtin_fma(tv, tv, br);
     const double Kr = mw3 * br;
     a[A_RX] = __builtin_fma(Kr, rx, a[A_RX]);
     a[A_RY] = __builtin_fma(Kr, ry, a[A_RY]);
     a[A_RZ] = __builtin_fma(Kr, rz, a[A_RZ]);
-    const double v4r = __builtin_fma(I.v4x, rx, __builtin_fma(I.v4y, ry, I.v4z * rz));
-    const double Ku = mw3 * __builtin_fma(-3.0, vjr, v4r);
-    a[A_UX] = __builtin_fma(Ku, ux, a[A_UX]);
-    a[A_UY] = __builtin_fma(Ku, uy, a[A_UY]);
-    a[A_UZ] = __builtin_fma(Ku, uz, a[A_UZ]);
+    const double gr = __builtin_fma(I.gx, rx, __builtin_fma(I.gy, ry, I.gz * rz));
+    const double Ku = mw3 * __builtin_fma(-PN1_SQ6, sr, gr);
+    a[A_UX] = __builtin_fma(Ku, J.sx, a[A_UX]);
+    a[A_UY] = __builtin_fma(Ku, J.sy, a[A_UY]);
+    a[A_UZ] = __builtin_fma(Ku, J.sz, a[A_UZ]);
+    a[A_KS] += Ku;
+}
+
+// j loop.  PP = 0: one record at a time.  PP = 1: two records in flight,
+// the scalar loads of one issued before the other is used; j + 1 < j1 <= n,
+// so jr[j + 2] stays within the cap + 2 records allocated.
+template <bool MASK, int PP>
+__device__ __forceinline__ void run64(const JRec64* __restrict__ jr, int j0, int j1, int i,
+                                      const I64& I, double (&a)[NACC], double eps2)
+{
+    if (PP == 0) {
+        for (int j = j0; j < j1; ++j) body64<MASK>(jr[j], j, i, I, a, eps2);
+        return;
+    }
+    if (j0 >= j1) return;
+    JRec64 ra = jr[j0];
+    int j = j0;
+    for (; j + 1 < j1; j += 2) {
+        const JRec64 rb = jr[j + 1];
+        body64<MASK>(ra, j, i, I, a, eps2);
+        ra = jr[j + 2];
+        body64<MASK>(rb, j + 1, i, I, a, eps2);
+    }
+    if (j < j1) body64<MASK>(ra, j, i, I, a, eps2);
 }
 
 // Chunk bounds of the j range [1, n) split in S parts, and the part of it
@@ -314,7 +355,9 @@ __device__ __forceinline__ JRange j_range(int s, int S, int n, int ib)
     return R;
 }
 
+template <int PP>
 __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
+                                              const double* __restrict__ st,
                                               double* __restrict__ part, Phys P, int S,
                                               int nIblk)
 {
@@ -324,18 +367,18 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I64 I = load_i64(jr, i, P.m0, P.eps2);
+    const I64 I = load_i64(jr, st, P.cap, i, P.m0, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const double eps2 = P.eps2;
     double a[NACC];
 #pragma unroll
     for (int k = 0; k < NACC; ++k) a[k] = 0.0;
     if (R.w0 < R.w1) {
-        for (int j = R.j0; j < R.w0; ++j) body64<false>(jr[j], j, i, I, a, eps2);
-        for (int j = R.w0; j < R.w1; ++j) body64<true>(jr[j], j, i, I, a, eps2);
-        for (int j = R.w1; j < R.j1; ++j) body64<false>(jr[j], j, i, I, a, eps2);
+        run64<false, PP>(jr, R.j0, R.w0, i, I, a, eps2);
+        run64<true, PP>(jr, R.w0, R.w1, i, I, a, eps2);
+        run64<false, PP>(jr, R.w1, R.j1, i, I, a, eps2);
     } else {
-        for (int j = R.j0; j < R.j1; ++j) body64<false>(jr[j], j, i, I, a, eps2);
+        run64<false, PP>(jr, R.j0, R.j1, i, I, a, eps2);
     }
     if (li < n) {
 #pragma unroll
@@ -348,23 +391,24 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
 // positions, FP32 tile sums folded into Kahan-compensated FP32 totals
 // ---------------------------------------------------------------------
 struct I32 {
-    float xh, yh, zh, xl, yl, zl, vx, vy, vz, v4x, v4y, v4z, al, D2, D2e, e;
+    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, al, D2, D2e;
 };
 
-__device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr, int i, float m0,
-                                        float eps2)
+__device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
+                                        const double* __restrict__ st, int cap, int i,
+                                        double m0, double eps2)
 {
-    const JRec32 r = jr[i];
+    const JRec32& r = jr[i];
+    const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
     I32 I;
     I.xh = r.xh; I.yh = r.yh; I.zh = r.zh;
     I.xl = r.xl; I.yl = r.yl; I.zl = r.zl;
-    I.vx = r.vx; I.vy = r.vy; I.vz = r.vz;
-    I.v4x = 4.0f * r.vx; I.v4y = 4.0f * r.vy; I.v4z = 4.0f * r.vz;
-    const float V2 = r.vx * r.vx + r.vy * r.vy + r.vz * r.vz;
-    I.al = 4.0f * m0 * r.e + V2;
+    I.gx = (float)(4.0 * vx); I.gy = (float)(4.0 * vy); I.gz = (float)(4.0 * vz);
+    I.hx = (float)(PN1_4OSQ15 * vx); I.hy = (float)(PN1_4OSQ15 * vy);
+    I.hz = (float)(PN1_4OSQ15 * vz);
+    I.al = (float)(4.0 * m0 * (double)r.e - (vx * vx + vy * vy + vz * vz));
     I.D2 = r.D2;
-    I.D2e = r.D2 + eps2;
-    I.e = r.e;
+    I.D2e = (float)((double)r.D2 + eps2);
     return I;
 }
 
@@ -372,11 +416,12 @@ template <bool MASK>
 __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32& I,
                                        float (&a)[NACC], float eps2)
 {
-    float mj = J.m, Qj = J.Q;
+    float mj = J.m, Qj = J.Q, MBj = J.MB;
     if (MASK) {
         const bool self = (j == i);
         mj = self ? 0.0f : mj;
         Qj = self ? 0.0f : Qj;
+        MBj = self ? 0.0f : MBj;
     }
     // separation from the hi and lo parts: close pairs keep their digits
     const float rx = (J.xh - I.xh) + (J.xl - I.xl);
@@ -392,31 +437,50 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
     a[A_XI] = __builtin_fmaf(mw3, I.D2 - J.D2, a[A_XI]);
-    const float c3 = Qj * (w - I.e);
-    a[A_TX] = __builtin_fmaf(c3, J.dx, a[A_TX]);
-    a[A_TY] = __builtin_fmaf(c3, J.dy, a[A_TY]);
-    a[A_TZ] = __builtin_fmaf(c3, J.dz, a[A_TZ]);
+    const float qw = Qj * w;
+    a[A_TX] = __builtin_fmaf(qw, J.dx, a[A_TX]);
+    a[A_TY] = __builtin_fmaf(qw, J.dy, a[A_TY]);
+    a[A_TZ] = __builtin_fmaf(qw, J.dz, a[A_TZ]);
     const float t = R2e - I.D2e;
-    a[A_ZE] = __builtin_fmaf(mj * J.B, t, a[A_ZE]);
-    const float ux = J.vx - I.vx, uy = J.vy - I.vy, uz = J.vz - I.vz;
-    const float uu = __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz));
-    const float vjr = __builtin_fmaf(J.vx, rx, __builtin_fmaf(J.vy, ry, J.vz * rz));
+    a[A_ZE] = __builtin_fmaf(MBj, t, a[A_ZE]);
+    const float sr = __builtin_fmaf(J.sx, rx, __builtin_fmaf(J.sy, ry, J.sz * rz));
     float br = __builtin_fmaf(J.B, t, I.al + J.A);
-    br = __builtin_fmaf(-2.0f, uu, br);
-    const float tv = vjr * w;
-    br = __builtin_fmaf(1.5f * tv, tv, br);
+    br = __builtin_fmaf(I.hx, J.sx, __builtin_fmaf(I.hy, J.sy, __builtin_fmaf(I.hz, J.sz, br)));
+    const float tv = sr * w;
+    br = __builtin_fmaf(tv, tv, br);
     const float Kr = mw3 * br;
     a[A_RX] = __builtin_fmaf(Kr, rx, a[A_RX]);
     a[A_RY] = __builtin_fmaf(Kr, ry, a[A_RY]);
     a[A_RZ] = __builtin_fmaf(Kr, rz, a[A_RZ]);
-    const float v4r = __builtin_fmaf(I.v4x, rx, __builtin_fmaf(I.v4y, ry, I.v4z * rz));
-    const float Ku = mw3 * __builtin_fmaf(-3.0f, vjr, v4r);
-    a[A_UX] = __builtin_fmaf(Ku, ux, a[A_UX]);
-    a[A_UY] = __builtin_fmaf(Ku, uy, a[A_UY]);
-    a[A_UZ] = __builtin_fmaf(Ku, uz, a[A_UZ]);
+    const float gr = __builtin_fmaf(I.gx, rx, __builtin_fmaf(I.gy, ry, I.gz * rz));
+    const float Ku = mw3 * __builtin_fmaf((float)-PN1_SQ6, sr, gr);
+    a[A_UX] = __builtin_fmaf(Ku, J.sx, a[A_UX]);
+    a[A_UY] = __builtin_fmaf(Ku, J.sy, a[A_UY]);
+    a[A_UZ] = __builtin_fmaf(Ku, J.sz, a[A_UZ]);
+    a[A_KS] += Ku;
 }
 
-// Kahan step: s + c carries the running total, c the negated lost part.
+template <bool MASK, int PP>
+__device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int j1, int i,
+                                      const I32& I, float (&a)[NACC], float eps2)
+{
+    if (PP == 0) {
+        for (int j = j0; j < j1; ++j) body32<MASK>(jr[j], j, i, I, a, eps2);
+        return;
+    }
+    if (j0 >= j1) return;
+    JRec32 ra = jr[j0];
+    int j = j0;
+    for (; j + 1 < j1; j += 2) {
+        const JRec32 rb = jr[j + 1];
+        body32<MASK>(ra, j, i, I, a, eps2);
+        ra = jr[j + 2];
+        body32<MASK>(rb, j + 1, i, I, a, eps2);
+    }
+    if (j < j1) body32<MASK>(ra, j, i, I, a, eps2);
+}
+
+// Kahan step: s - c carries the running total, c the lost low-order part.
 __device__ __forceinline__ void kahan_add(float& s, float& c, float t)
 {
 #pragma clang fp contract(off)
@@ -426,7 +490,9 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float t)
     s = u;
 }
 
+template <int PP>
 __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
+                                              const double* __restrict__ st,
                                               float* __restrict__ part, Phys P, int S,
                                               int nIblk, int tile)
 {
@@ -436,7 +502,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(jr, i, P.m0f, P.eps2f);
+    const I32 I = load_i32(jr, st, P.cap, i, P.m0, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
     float sum[NACC], cmp[NACC];
@@ -447,11 +513,10 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
         float a[NACC];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
-        if (jt < R.w1 && je > R.w0) {
-            for (int j = jt; j < je; ++j) body32<true>(jr[j], j, i, I, a, eps2);
-        } else {
-            for (int j = jt; j < je; ++j) body32<false>(jr[j], j, i, I, a, eps2);
-        }
+        if (jt < R.w1 && je > R.w0)
+            run32<true, PP>(jr, jt, je, i, I, a, eps2);
+        else
+            run32<false, PP>(jr, jt, je, i, I, a, eps2);
 #pragma unroll
         for (int k = 0; k < NACC; ++k) kahan_add(sum[k], cmp[k], a[k]);
     }
@@ -518,6 +583,7 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
         double S_[NACC];
         gather_acc<MODE>(part, S, cap, p, S_);
         const double G1 = gs[G_G1];
+        const double M1[3] = {gs[G_M1X], gs[G_M1Y], gs[G_M1Z]};
         const double Pm[3] = {gs[G_PX], gs[G_PY], gs[G_PZ]};
         const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
         const double vd = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
@@ -536,7 +602,9 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
             const double B = -m0 * E3 * (d[k] * cB + 4.0 * vd * v[k]);
             const double Sg = cS * d[k] + 7.0 * mp * E3 * vd * v[k];
             const double t4 = E3 * (4.0 * vP * d[k] - 3.0 * Pd * v[k] - 4.0 * vd * Pp[k]);
-            const double pn = B + Sg + S_[A_RX + k] + S_[A_UX + k] + c1 * d[k] + S_[A_TX + k] + t4;
+            const double ru = S_[A_UX + k] / PN1_SQ15 - v[k] * S_[A_KS];
+            const double t3 = S_[A_TX + k] - 3.5 * m0 * e * (M1[k] - q * d[k]);
+            const double pn = B + Sg + S_[A_RX + k] + ru + c1 * d[k] + t3 + t4;
             a[k] = newton + P.lam * pn;
             acc[(size_t)k * cap + p] = a[k];
         }
@@ -599,8 +667,15 @@ __global__ __launch_bounds__(WG) void k_bh(const double* __restrict__ bhblk, int
 // ---------------------------------------------------------------------
 // energy: pair sums (FP64 in both modes), per-star terms, totals
 // ---------------------------------------------------------------------
+// Energy pair body.  The record holds s_j = sqrt(3/2) v_j; the lane's
+// velocity is pre-divided by sqrt(3/2) so that (r.v_i)(r.s_j) needs no
+// rescaling, and k_efin divides sum m_j w s_j by sqrt(3/2).
+struct IE64 {
+    double x, y, z, ux, uy, uz;
+};
+
 template <bool MASK>
-__device__ __forceinline__ void ebody64(const JRec64& J, int j, int i, const I64& I,
+__device__ __forceinline__ void ebody64(const JRec64& J, int j, int i, const IE64& I,
                                         double (&a)[NACCE], double eps2)
 {
     double mj = J.m;
@@ -610,40 +685,58 @@ __device__ __forceinline__ void ebody64(const JRec64& J, int j, int i, const I64
     const double w = rsq64(R2e);
     const double mw = mj * w;
     a[E_PHI] += mw;
-    a[E_PVX] = __builtin_fma(mw, J.vx, a[E_PVX]);
-    a[E_PVY] = __builtin_fma(mw, J.vy, a[E_PVY]);
-    a[E_PVZ] = __builtin_fma(mw, J.vz, a[E_PVZ]);
-    const double rvi = __builtin_fma(I.vx, rx, __builtin_fma(I.vy, ry, I.vz * rz));
-    const double rvj = __builtin_fma(J.vx, rx, __builtin_fma(J.vy, ry, J.vz * rz));
+    a[E_PVX] = __builtin_fma(mw, J.sx, a[E_PVX]);
+    a[E_PVY] = __builtin_fma(mw, J.sy, a[E_PVY]);
+    a[E_PVZ] = __builtin_fma(mw, J.sz, a[E_PVZ]);
+    const double rvi = __builtin_fma(I.ux, rx, __builtin_fma(I.uy, ry, I.uz * rz));
+    const double rvj = __builtin_fma(J.sx, rx, __builtin_fma(J.sy, ry, J.sz * rz));
     a[E_GAM] = __builtin_fma(mw * (w * w), rvi * rvj, a[E_GAM]);
 }
 
+template <bool MASK>
+__device__ __forceinline__ void erun64(const JRec64* __restrict__ jr, int j0, int j1, int i,
+                                       const IE64& I, double (&a)[NACCE], double eps2)
+{
+    if (j0 >= j1) return;
+    JRec64 cur = jr[j0];
+    for (int j = j0; j < j1; ++j) {
+        const JRec64 nxt = jr[j + 1];
+        ebody64<MASK>(cur, j, i, I, a, eps2);
+        cur = nxt;
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
+                                               const double* __restrict__ st,
                                                double* __restrict__ part, Phys P, int S,
                                                int nIblk)
 {
     int s, ib;
     map_block(blockIdx.x, S, s, ib);
     if (ib >= nIblk) return;
-    const int n = P.n;
+    const int n = P.n, cap = P.cap;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I64 I = load_i64(jr, i, P.m0, P.eps2);
+    IE64 I;
+    I.x = jr[i].x; I.y = jr[i].y; I.z = jr[i].z;
+    I.ux = st[3 * cap + i] / PN1_SQ15;
+    I.uy = st[4 * cap + i] / PN1_SQ15;
+    I.uz = st[5 * cap + i] / PN1_SQ15;
     const JRange R = j_range(s, S, n, ib);
     const double eps2 = P.eps2;
     double a[NACCE];
 #pragma unroll
     for (int k = 0; k < NACCE; ++k) a[k] = 0.0;
     if (R.w0 < R.w1) {
-        for (int j = R.j0; j < R.w0; ++j) ebody64<false>(jr[j], j, i, I, a, eps2);
-        for (int j = R.w0; j < R.w1; ++j) ebody64<true>(jr[j], j, i, I, a, eps2);
-        for (int j = R.w1; j < R.j1; ++j) ebody64<false>(jr[j], j, i, I, a, eps2);
+        erun64<false>(jr, R.j0, R.w0, i, I, a, eps2);
+        erun64<true>(jr, R.w0, R.w1, i, I, a, eps2);
+        erun64<false>(jr, R.w1, R.j1, i, I, a, eps2);
     } else {
-        for (int j = R.j0; j < R.j1; ++j) ebody64<false>(jr[j], j, i, I, a, eps2);
+        erun64<false>(jr, R.j0, R.j1, i, I, a, eps2);
     }
     if (li < n) {
 #pragma unroll
-        for (int k = 0; k < NACCE; ++k) part[(size_t)(s * NACCE + k) * P.cap + li] = a[k];
+        for (int k = 0; k < NACCE; ++k) part[(size_t)(s * NACCE + k) * cap + li] = a[k];
     }
 }
 
@@ -677,7 +770,7 @@ __global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
         const double v0v = v0[0] * v[0] + v0[1] * v[1] + v0[2] * v[2];
         const double dv0 = d[0] * v0[0] + d[1] * v0[1] + d[2] * v0[2];
         const double dv = d[0] * v[0] + d[1] * v[1] + d[2] * v[2];
-        const double vPhv = v[0] * A[E_PVX] + v[1] * A[E_PVY] + v[2] * A[E_PVZ];
+        const double vPhv = (v[0] * A[E_PVX] + v[1] * A[E_PVY] + v[2] * A[E_PVZ]) / PN1_SQ15;
         const double G1 = gs[G_G1];
         t[0] = mp * V2;
         t[1] = -0.5 * mp * A[E_PHI];
@@ -714,6 +807,15 @@ __global__ __launch_bounds__(WG) void k_ered(const double* __restrict__ eblk, in
         out[3] = pn2;
         out[4] = kin + pot + pn + pn2;
     }
+}
+
+// Plain streaming copy (16 B per lane per iteration): the reference
+// workload for calibrating rocprof's HBM byte counters.
+__global__ __launch_bounds__(WG) void k_copy(const double2* __restrict__ src,
+                                            double2* __restrict__ dst, size_t n2)
+{
+    const size_t stride = (size_t)gridDim.x * WG;
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
 }
 
 }  // namespace pn1dev

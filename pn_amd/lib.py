@@ -63,10 +63,11 @@ def load():
         "pn1_step": (_I, [P, _I]),
         "pn1_sync": (_I, [P]),
         "pn1_last_error": (ctypes.c_char_p, [P]),
-        "pn1_set_tuning": (_I, [P, _I, _I]),
-        "pn1_get_tuning": (_I, [P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        "pn1_set_tuning": (_I, [P, _I, _I, _I]),
+        "pn1_get_tuning": (_I, [P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "pn1_profile_enable": (_I, [P, _I]),
         "pn1_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
+        "pn1_util_copy": (_I, [P, ctypes.c_longlong, _I]),
         "pn1_check_input": (_I, [_D, _D, _D, _I, ctypes.c_double, ctypes.c_char_p, _I]),
         "pn1_ic_uniform_sphere": (_I, [_I, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, _D, _D, _D]),
@@ -136,14 +137,15 @@ class Context:
     def set_params(self, eps, c=1.0, dt=2.0**-10, mode=FP64):
         self._chk(self._L.pn1_set_params(self._h, eps, c, dt, int(mode)), "pn1_set_params")
 
-    def set_tuning(self, jsplit=0, tile=0):
-        self._chk(self._L.pn1_set_tuning(self._h, int(jsplit), int(tile)), "pn1_set_tuning")
+    def set_tuning(self, jsplit=0, tile=0, variant=0):
+        self._chk(self._L.pn1_set_tuning(self._h, int(jsplit), int(tile), int(variant)),
+                  "pn1_set_tuning")
 
     def get_tuning(self):
-        s, t = _I(), _I()
-        self._chk(self._L.pn1_get_tuning(self._h, ctypes.byref(s), ctypes.byref(t)),
-                  "pn1_get_tuning")
-        return s.value, t.value
+        s, t, v = _I(), _I(), _I()
+        self._chk(self._L.pn1_get_tuning(self._h, ctypes.byref(s), ctypes.byref(t),
+                                         ctypes.byref(v)), "pn1_get_tuning")
+        return s.value, t.value, v.value
 
     def upload(self, x, v, m):
         m = _f64(m)
@@ -176,6 +178,9 @@ class Context:
 
     def profile_enable(self, on=True):
         self._chk(self._L.pn1_profile_enable(self._h, 1 if on else 0), "pn1_profile_enable")
+
+    def util_copy(self, nbytes, reps=1):
+        self._chk(self._L.pn1_util_copy(self._h, int(nbytes), int(reps)), "pn1_util_copy")
 
     def profile_read(self):
         k, ms = _I(), ctypes.c_double()

@@ -16,13 +16,15 @@ Pair sums (per star i), the only O(n^2) work:
   Nst   = sum' m_j w^3 r                       (star-star Newton)
   PNr   = sum' K_r r,  K_r = m_j w^3 [m0 (4 e_i + 1.25 e_j + 0.25 e_j^3 (|r|^2 - D2_i))
                                       + V2_i - 2|u|^2 + 1.5 (v_j.r)^2 w^2]
-  PNu   = sum' m_j w^3 (r.(4 v_i - 3 v_j)) u
-  T3    = sum' 3.5 m0 q_j (w - e_i) d_j
+          with V2_i - 2|u|^2 = -V2_i - 2 V2_j + 4 v_i.v_j
+  RU, KS = sum' K_u v_j, sum' K_u,  K_u = m_j w^3 (r.(4 v_i - 3 v_j)),
+          so that sum' K_u u = RU - v_i KS
+  T3    = sum' 3.5 m0 q_j w d_j   (the O(n) pass subtracts e_i sum' 3.5 m0 q_j d_j)
   phi   = sum' m_j w,  Xi = sum' m_j w^3 (D2_i - D2_j)
   zeta  = sum' m_j 0.25 m0 e_j^3 (|r|^2 - D2_i)
-T3 keeps the reference's grouping (w - e_i) d_j (Limit-sticky9.c:307-308):
-splitting it into sum q_j w r + d_i sum q_j w - e_i sum q_j d_j cancels
-catastrophically when a star sits close to the BH (q_j = m_j e_j^3 large).
+The (w - e_i) d_j term (Limit-sticky9.c:307-308) is summed over d_j, the
+star-BH separation: re-expressing d_j as r + d_i instead would turn it
+into large sums that cancel when a star sits close to the BH.
 zeta and the 0.25 e_j^3 (|r|^2 - D2_i) part of K_r cancel each other for a
 star j near the BH; both take the same per-pair |r|^2 - D2_i, as the
 reference's two terms take the same r1b2 - rbc2 (Limit-sticky9.c:302, 305),
@@ -50,29 +52,29 @@ def pair_sums(x, v, m, eps):
     R2 = np.einsum("ijk,ijk->ij", r, r)
     w = 1.0 / np.sqrt(R2 + eps * eps)
     np.fill_diagonal(w, 0.0)                       # skip j = i
+    live = w > 0
     w2 = w * w
     mw = ms[None, :] * w
     mw3 = mw * w2
-    u = vs[None, :, :] - vs[:, None, :]
-    uu = np.einsum("ijk,ijk->ij", u, u)
-    vjr = np.einsum("jk,ijk->ij", vs, r)
     V2 = np.einsum("ij,ij->i", vs, vs)
+    vivj = vs @ vs.T
+    vjr = np.einsum("jk,ijk->ij", vs, r)
     t = R2 - D2[:, None]
-    br = (m0 * (4 * e[:, None] + 1.25 * e[None, :]
-                + 0.25 * e[None, :] ** 3 * t)
-          + V2[:, None] - 2 * uu + 1.5 * vjr**2 * w2)
+    # V2_i - 2|u|^2 = -V2_i - 2 V2_j + 4 v_i.v_j
+    br = ((4 * m0 * e - V2)[:, None] + (1.25 * m0 * e - 2 * V2)[None, :]
+          + (0.25 * m0 * e**3)[None, :] * t + 4 * vivj + 1.5 * vjr**2 * w2)
     Kr = mw3 * br
-    c3 = np.where(w > 0, 3.5 * m0 * q[None, :] * (w - e[:, None]), 0.0)
-    vir4 = 4 * np.einsum("ik,ijk->ij", vs, r)
-    Ku = mw3 * (vir4 - 3 * vjr)
+    Ku = mw3 * (4 * np.einsum("ik,ijk->ij", vs, r) - 3 * vjr)
+    qw = 3.5 * m0 * q[None, :] * w
     return dict(
         Nst=np.einsum("ij,ijk->ik", mw3, r),
         PNr=np.einsum("ij,ijk->ik", Kr, r),
-        PNu=np.einsum("ij,ijk->ik", Ku, u),
+        RU=Ku @ vs,
+        KS=Ku.sum(1),
+        T3=qw @ d,
         phi=mw.sum(1),
-        T3=np.einsum("ij,jk->ik", c3, d),
         Xi=(mw3 * (D2[:, None] - D2[None, :])).sum(1),
-        zeta=np.where(w > 0, (0.25 * m0 * q)[None, :] * t, 0.0).sum(1),
+        zeta=np.where(live, (0.25 * m0 * q)[None, :] * t, 0.0).sum(1),
     )
 
 
@@ -104,7 +106,9 @@ def force(x, v, m, eps, c):
     vP = np.einsum("ij,ij->i", vs, Pp)
     Pd = np.einsum("ij,ij->i", Pp, d)
     t4 = E3[:, None] * (4 * vP[:, None] * d - 3 * Pd[:, None] * vs - 4 * vd[:, None] * Pp)
-    X = S["PNr"] + S["PNu"] + t1 + S["T3"] + t4
+    PNu = S["RU"] - vs * S["KS"][:, None]
+    T3 = S["T3"] - (3.5 * m0 * e)[:, None] * (M1[None, :] - q[:, None] * d)
+    X = S["PNr"] + PNu + t1 + T3 + t4
     a[1:] = newton + lam * (B + Sg + X)
 
     # black hole
