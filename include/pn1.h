@@ -104,8 +104,10 @@ const char *pn1_last_error(const pn1_ctx *ctx);
 
 /* Launch shape of the pair kernels: the j range is split into `jsplit`
  * chunks (0 = automatic), the mixed kernel compensates its FP32 sums every
- * `tile` pairs (0 = default), and `variant` picks the j-loop form (0: one
- * j-record at a time, 1: two in flight).  Used by the tuner. */
+ * `tile` pairs (0 = default), and `variant` picks the kernel form (-1 =
+ * automatic per mode): bit 0 the j loop (clear: one j-record at a time,
+ * set: two in flight), bit 1 (mixed form) one Newton step on the FP32
+ * rsqrt.  pn1_get_tuning reports the values in use.  Used by the tuner. */
 int  pn1_set_tuning(pn1_ctx *ctx, int jsplit, int tile, int variant);
 int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile, int *variant);
 

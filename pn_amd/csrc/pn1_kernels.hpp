@@ -45,7 +45,9 @@ constexpr int NEB = 4;      // energy per-block partials
 //   N    = sum m_j w^3 r                                      (Newton)
 //   Rr   = sum K_r r,  K_r = m_j w^3 [m0(4e_i + 1.25e_j + 0.25e_j^3 t)
 //                                      + |v_i|^2 - 2|u|^2 + 1.5 (v_j.r)^2 w^2]
-//          using |v_i|^2 - 2|u|^2 = -|v_i|^2 - 2|v_j|^2 + 4 v_i.v_j
+//          using |v_i|^2 - 2|u|^2 = -|v_i|^2 - 2|v_j|^2 + 4 v_i.v_j; the
+//          i-only part al_i = 4 m0 e_i - |v_i|^2 is left out of the pair sum
+//          and added by k_fin as al_i N
 //   U    = sum K_u s_j,  KS = sum K_u      (sum K_u u = U / sqrt(3/2) - v_i KS)
 //   T3   = sum 3.5 m0 m_j e_j^3 w d_j      (k_fin subtracts e_i times its w-free sum)
 //   phi  = sum m_j w,  xi = sum m_j w^3 (D2_i - D2_j),  zeta = sum m_j 0.25 m0 e_j^3 t
@@ -248,9 +250,9 @@ __global__ __launch_bounds__(WG) void k_rows_sum(const double* __restrict__ rows
 // force pair kernel, FP64
 // ---------------------------------------------------------------------
 // Per-lane state of the i particle: position, g = 4 v, h = 4 v / sqrt(3/2),
-// al = 4 m0 e - |v|^2, |d|^2 and |d|^2 + eps^2.
+// |d|^2 and |d|^2 + eps^2.
 struct I64 {
-    double x, y, z, gx, gy, gz, hx, hy, hz, al, D2, D2e;
+    double x, y, z, gx, gy, gz, hx, hy, hz, D2, D2e;
 };
 
 __device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr,
@@ -263,7 +265,6 @@ __device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr,
     I.x = r.x; I.y = r.y; I.z = r.z;
     I.gx = 4.0 * vx; I.gy = 4.0 * vy; I.gz = 4.0 * vz;
     I.hx = PN1_4OSQ15 * vx; I.hy = PN1_4OSQ15 * vy; I.hz = PN1_4OSQ15 * vz;
-    I.al = 4.0 * m0 * r.e - (vx * vx + vy * vy + vz * vz);
     I.D2 = r.D2;
     I.D2e = r.D2 + eps2;
     return I;
@@ -298,7 +299,7 @@ __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64&
     const double t = R2e - I.D2e;
     a[A_ZE] = __builtin_fma(MBj, t, a[A_ZE]);
     const double sr = __builtin_fma(J.sx, rx, __builtin_fma(J.sy, ry, J.sz * rz));
-    double br = __builtin_fma(J.B, t, I.al + J.A);
+    double br = __builtin_fma(J.B, t, J.A);
     br = __builtin_fma(I.hx, J.sx, __builtin_fma(I.hy, J.sy, __builtin_fma(I.hz, J.sz, br)));
     const double tv = sr * w;
     br = __builtin_fma(tv, tv, br);
@@ -314,14 +315,15 @@ __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64&
     a[A_KS] += Ku;
 }
 
-// j loop.  PP = 0: one record at a time.  PP = 1: two records in flight,
-// the scalar loads of one issued before the other is used; j + 1 < j1 <= n,
-// so jr[j + 2] stays within the cap + 2 records allocated.
+// j loop.  PP bit 0 clear: one record at a time; set: two records in
+// flight, the scalar loads of one issued before the other is used (j + 1 <
+// j1 <= n, so jr[j + 2] stays within the cap + 2 records allocated).
+// PP bit 1 (mixed form only): one Newton step on the FP32 rsqrt.
 template <bool MASK, int PP>
 __device__ __forceinline__ void run64(const JRec64* __restrict__ jr, int j0, int j1, int i,
                                       const I64& I, double (&a)[NACC], double eps2)
 {
-    if (PP == 0) {
+    if ((PP & 1) == 0) {
         for (int j = j0; j < j1; ++j) body64<MASK>(jr[j], j, i, I, a, eps2);
         return;
     }
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
 // positions, FP32 tile sums folded into Kahan-compensated FP32 totals
 // ---------------------------------------------------------------------
 struct I32 {
-    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, al, D2, D2e;
+    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2, D2e;
 };
 
 __device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
@@ -406,13 +408,12 @@ __device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
     I.gx = (float)(4.0 * vx); I.gy = (float)(4.0 * vy); I.gz = (float)(4.0 * vz);
     I.hx = (float)(PN1_4OSQ15 * vx); I.hy = (float)(PN1_4OSQ15 * vy);
     I.hz = (float)(PN1_4OSQ15 * vz);
-    I.al = (float)(4.0 * m0 * (double)r.e - (vx * vx + vy * vy + vz * vz));
     I.D2 = r.D2;
     I.D2e = (float)((double)r.D2 + eps2);
     return I;
 }
 
-template <bool MASK>
+template <bool MASK, bool RF>
 __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32& I,
                                        float (&a)[NACC], float eps2)
 {
@@ -428,7 +429,12 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     const float ry = (J.yh - I.yh) + (J.yl - I.yl);
     const float rz = (J.zh - I.zh) + (J.zl - I.zl);
     const float R2e = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
-    const float w = __builtin_amdgcn_rsqf(R2e);
+    float w = __builtin_amdgcn_rsqf(R2e);
+    if (RF) {
+        // one Newton step: v_rsq_f32 (~1.6 ulp) -> ~1 ulp
+        const float r = __builtin_fmaf(-(R2e * w), w, 1.0f);
+        w = __builtin_fmaf(0.5f * w, r, w);
+    }
     const float w2 = w * w;
     const float mw = mj * w;
     const float mw3 = mw * w2;
@@ -444,7 +450,7 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     const float t = R2e - I.D2e;
     a[A_ZE] = __builtin_fmaf(MBj, t, a[A_ZE]);
     const float sr = __builtin_fmaf(J.sx, rx, __builtin_fmaf(J.sy, ry, J.sz * rz));
-    float br = __builtin_fmaf(J.B, t, I.al + J.A);
+    float br = __builtin_fmaf(J.B, t, J.A);
     br = __builtin_fmaf(I.hx, J.sx, __builtin_fmaf(I.hy, J.sy, __builtin_fmaf(I.hz, J.sz, br)));
     const float tv = sr * w;
     br = __builtin_fmaf(tv, tv, br);
@@ -460,12 +466,37 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     a[A_KS] += Ku;
 }
 
+// Folds the short-run FP32 Newtonian sums into their Kahan totals.
+__device__ __forceinline__ void fold_newton(float (&a)[NACC], float (&sN)[3], float (&cN)[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma clang fp contract(off)
+        const float y = a[A_NX + k] - cN[k];
+        const float u = sN[k] + y;
+        cN[k] = (u - sN[k]) - y;
+        sN[k] = u;
+        a[A_NX + k] = 0.0f;
+    }
+}
+
+// Mixed-form j loop: as run64, plus a wave-uniform pair counter that folds
+// the Newtonian sums into Kahan totals every `tile` pairs without
+// restarting the record stream.
 template <bool MASK, int PP>
 __device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int j1, int i,
-                                      const I32& I, float (&a)[NACC], float eps2)
+                                      const I32& I, float (&a)[NACC], float eps2,
+                                      float (&sN)[3], float (&cN)[3], int tile, int& cnt)
 {
-    if (PP == 0) {
-        for (int j = j0; j < j1; ++j) body32<MASK>(jr[j], j, i, I, a, eps2);
+    constexpr bool RF = (PP & 2) != 0;
+    if ((PP & 1) == 0) {
+        for (int j = j0; j < j1; ++j) {
+            body32<MASK, RF>(jr[j], j, i, I, a, eps2);
+            if (++cnt >= tile) {
+                cnt = 0;
+                fold_newton(a, sN, cN);
+            }
+        }
         return;
     }
     if (j0 >= j1) return;
@@ -473,23 +504,28 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int
     int j = j0;
     for (; j + 1 < j1; j += 2) {
         const JRec32 rb = jr[j + 1];
-        body32<MASK>(ra, j, i, I, a, eps2);
+        body32<MASK, RF>(ra, j, i, I, a, eps2);
         ra = jr[j + 2];
-        body32<MASK>(rb, j + 1, i, I, a, eps2);
+        body32<MASK, RF>(rb, j + 1, i, I, a, eps2);
+        cnt += 2;
+        if (cnt >= tile) {
+            cnt = 0;
+            fold_newton(a, sN, cN);
+        }
     }
-    if (j < j1) body32<MASK>(ra, j, i, I, a, eps2);
+    if (j < j1) {
+        body32<MASK, RF>(ra, j, i, I, a, eps2);
+        ++cnt;
+    }
 }
 
-// Kahan step: s - c carries the running total, c the lost low-order part.
-__device__ __forceinline__ void kahan_add(float& s, float& c, float t)
-{
-#pragma clang fp contract(off)
-    const float y = t - c;
-    const float u = s + y;
-    c = (u - s) - y;
-    s = u;
-}
 
+// The Newtonian sums are Kahan-compensated every `tile` pairs (at least
+// 2): they carry the stars' accelerations, and plain FP32 sums over longer
+// runs are the largest error of this form (measured at n = 131073: max
+// per-star error 8e-7 compensating every 32 pairs, 1.5e-6 every 256).  The other sums
+// enter scaled by 1/c^2 (or, for the BH, beside its FP64 Newtonian term)
+// and accumulate in plain FP32 over the whole chunk.
 template <int PP>
 __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
                                               const double* __restrict__ st,
@@ -505,27 +541,27 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const I32 I = load_i32(jr, st, P.cap, i, P.m0, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
-    float sum[NACC], cmp[NACC];
+    float a[NACC], sumN[3], cmpN[3];
 #pragma unroll
-    for (int k = 0; k < NACC; ++k) sum[k] = cmp[k] = 0.0f;
-    for (int jt = R.j0; jt < R.j1; jt += tile) {
-        const int je = min(jt + tile, R.j1);
-        float a[NACC];
+    for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
-        if (jt < R.w1 && je > R.w0)
-            run32<true, PP>(jr, jt, je, i, I, a, eps2);
-        else
-            run32<false, PP>(jr, jt, je, i, I, a, eps2);
-#pragma unroll
-        for (int k = 0; k < NACC; ++k) kahan_add(sum[k], cmp[k], a[k]);
+    for (int k = 0; k < 3; ++k) sumN[k] = cmpN[k] = 0.0f;
+    int cnt = 0;
+    if (R.w0 < R.w1) {
+        run32<false, PP>(jr, R.j0, R.w0, i, I, a, eps2, sumN, cmpN, tile, cnt);
+        run32<true, PP>(jr, R.w0, R.w1, i, I, a, eps2, sumN, cmpN, tile, cnt);
+        run32<false, PP>(jr, R.w1, R.j1, i, I, a, eps2, sumN, cmpN, tile, cnt);
+    } else {
+        run32<false, PP>(jr, R.j0, R.j1, i, I, a, eps2, sumN, cmpN, tile, cnt);
     }
+    fold_newton(a, sumN, cmpN);
     if (li < n) {
         const size_t cap = (size_t)P.cap;
 #pragma unroll
         for (int k = 0; k < NACC; ++k) {
-            part[((size_t)(s * NACC + k) * 2 + 0) * cap + li] = sum[k];
-            part[((size_t)(s * NACC + k) * 2 + 1) * cap + li] = cmp[k];
+            const bool nk = k <= A_NZ;
+            part[((size_t)(s * NACC + k) * 2 + 0) * cap + li] = nk ? sumN[k] : a[k];
+            part[((size_t)(s * NACC + k) * 2 + 1) * cap + li] = nk ? cmpN[k] : 0.0f;
         }
     }
 }
@@ -593,7 +629,7 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
         const double vP = v[0] * Pp[0] + v[1] * Pp[1] + v[2] * Pp[2];
         const double Pd = Pp[0] * d[0] + Pp[1] * d[1] + Pp[2] * d[2];
         const double c1 = -m0 * E3 * (4.0 * S_[A_PHI] + 1.25 * (G1 - mp * e)) + E3 * S_[A_ZE];
-        const double cB = 4.0 * m0 * e - V2;
+        const double cB = 4.0 * m0 * e - V2;   // also the i-only part of the K_r bracket
         const double cS = 5.0 * m0 * mp * e * e * e * e - 4.0 * mp * E3 * V2;
         double a[3];
 #pragma unroll
@@ -604,7 +640,8 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
             const double t4 = E3 * (4.0 * vP * d[k] - 3.0 * Pd * v[k] - 4.0 * vd * Pp[k]);
             const double ru = S_[A_UX + k] / PN1_SQ15 - v[k] * S_[A_KS];
             const double t3 = S_[A_TX + k] - 3.5 * m0 * e * (M1[k] - q * d[k]);
-            const double pn = B + Sg + S_[A_RX + k] + ru + c1 * d[k] + t3 + t4;
+            const double rr = S_[A_RX + k] + cB * S_[A_NX + k];
+            const double pn = B + Sg + rr + ru + c1 * d[k] + t3 + t4;
             a[k] = newton + P.lam * pn;
             acc[(size_t)k * cap + p] = a[k];
         }

@@ -137,7 +137,7 @@ class Context:
     def set_params(self, eps, c=1.0, dt=2.0**-10, mode=FP64):
         self._chk(self._L.pn1_set_params(self._h, eps, c, dt, int(mode)), "pn1_set_params")
 
-    def set_tuning(self, jsplit=0, tile=0, variant=0):
+    def set_tuning(self, jsplit=0, tile=0, variant=-1):
         self._chk(self._L.pn1_set_tuning(self._h, int(jsplit), int(tile), int(variant)),
                   "pn1_set_tuning")
 

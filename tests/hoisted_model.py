@@ -61,7 +61,9 @@ def pair_sums(x, v, m, eps):
     vjr = np.einsum("jk,ijk->ij", vs, r)
     t = R2 - D2[:, None]
     # V2_i - 2|u|^2 = -V2_i - 2 V2_j + 4 v_i.v_j
-    br = ((4 * m0 * e - V2)[:, None] + (1.25 * m0 * e - 2 * V2)[None, :]
+    # the i-only part al_i = 4 m0 e_i - V2_i of the bracket is added in the
+    # O(n) pass as al_i * Nst
+    br = ((1.25 * m0 * e - 2 * V2)[None, :]
           + (0.25 * m0 * e**3)[None, :] * t + 4 * vivj + 1.5 * vjr**2 * w2)
     Kr = mw3 * br
     Ku = mw3 * (4 * np.einsum("ik,ijk->ij", vs, r) - 3 * vjr)
@@ -108,7 +110,8 @@ def force(x, v, m, eps, c):
     t4 = E3[:, None] * (4 * vP[:, None] * d - 3 * Pd[:, None] * vs - 4 * vd[:, None] * Pp)
     PNu = S["RU"] - vs * S["KS"][:, None]
     T3 = S["T3"] - (3.5 * m0 * e)[:, None] * (M1[None, :] - q[:, None] * d)
-    X = S["PNr"] + PNu + t1 + T3 + t4
+    PNr = S["PNr"] + (4 * m0 * e - V2)[:, None] * S["Nst"]
+    X = PNr + PNu + t1 + T3 + t4
     a[1:] = newton + lam * (B + Sg + X)
 
     # black hole

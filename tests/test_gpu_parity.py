@@ -12,9 +12,9 @@ Tolerances:
   FP64 form, energies: energy_atol (self-pair rounding of the reference).
   Mixed form: against the FP64 form, per-particle relative error
     p99.9 <= 1e-6 and max <= 1e-5 (stars), BH <= 1e-6, for c >= 50.  At
-    c = 1 (the reference's own units) the 1PN three-body terms, which the
-    mixed form evaluates in FP32, are as large as the Newtonian ones, and
-    the bound is p99.9 <= 3e-6.
+    c = 1 (the reference's own units) the 1PN three-body terms are as large
+    as the Newtonian ones; the mixed form compensates only its Newtonian
+    sums (the 1PN sums enter scaled by 1/c^2), and the bound is p99.9 <= 5e-6.
 """
 import numpy as np
 import pytest
@@ -137,6 +137,6 @@ def test_mixed_vs_fp64(ctx, name):
         stars = err[1:]
         print(f"{name} c={c}: mixed star p99.9 {np.quantile(stars, 0.999):.3e} "
               f"max {stars.max():.3e} BH {err[0]:.3e}")
-        assert np.quantile(stars, 0.999) <= (3e-6 if c == 1 else 1e-6)
+        assert np.quantile(stars, 0.999) <= (5e-6 if c == 1 else 1e-6)
         assert stars.max() <= 1e-5
         assert err[0] <= 1e-6
