@@ -131,7 +131,15 @@ def main():
     ap.add_argument("--jsplit", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel over MIB MiB, 3 times (for "
+                         "calibrating profiler byte counters), and exit")
     args = ap.parse_args()
+    if args.calib_copy:
+        with pn.Context(capacity=1, device=0) as ctx:
+            ctx.util_copy(args.calib_copy << 20, reps=3)
+        print(json.dumps({"calib_copy_bytes": args.calib_copy << 20, "reps": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this path runs on one GPU; there is no multi-GPU mode "
               f"(asked for --gpus {args.gpus}).", file=sys.stderr)

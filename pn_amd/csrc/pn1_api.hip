@@ -7,7 +7,8 @@
 //   H1, H2      stage states 1 and 2 (the reference's hx[1..2], hv[1..2])
 //   m[cap], acc[3][cap]
 //   j64[cap] / j32[cap]   per-evaluation j-records (AoS)
-//   part        pair partials [S][12][cap] doubles, or [S][12][2][cap] floats
+//   part        pair partials: FP64 [S][16][cap]; mixed [S][16 + 3][cap] floats
+//               (the sums, then the Kahan compensations of the Newtonian sums)
 // Nothing is allocated after pn1_upload / pn1_set_tuning: pn1_force,
 // pn1_energy and pn1_step only launch kernels and copy results.
 #include <hip/hip_runtime.h>
@@ -133,7 +134,7 @@ int ensure_part(pn1_ctx* c)
     const size_t cap = (size_t)c->cap, S = (size_t)c->S;
     size_t need = S * NACC * cap * sizeof(double);
     const size_t needE = S * NACCE * cap * sizeof(double);
-    const size_t need32 = S * NACC * 2 * cap * sizeof(float);
+    const size_t need32 = S * (NACC + 3) * cap * sizeof(float);
     if (needE > need) need = needE;
     if (need32 > need) need = need32;
     if (need <= c->part_bytes) return PN1_OK;

@@ -556,13 +556,13 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     }
     fold_newton(a, sumN, cmpN);
     if (li < n) {
+        // [s][NACC + 3][cap]: the sums, then the Newtonian compensations
         const size_t cap = (size_t)P.cap;
+        float* ps = part + (size_t)s * (NACC + 3) * cap + li;
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) {
-            const bool nk = k <= A_NZ;
-            part[((size_t)(s * NACC + k) * 2 + 0) * cap + li] = nk ? sumN[k] : a[k];
-            part[((size_t)(s * NACC + k) * 2 + 1) * cap + li] = nk ? cmpN[k] : 0.0f;
-        }
+        for (int k = 0; k < NACC; ++k) ps[k * cap] = k <= A_NZ ? sumN[k] : a[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ps[(NACC + k) * cap] = cmpN[k];
     }
 }
 
@@ -582,13 +582,13 @@ __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S,
             for (int k = 0; k < NACC; ++k) A[k] += pp[(size_t)(s * NACC + k) * cap + p];
     } else {
         const float* pp = (const float*)part;
-        for (int s = 0; s < S; ++s)
+        for (int s = 0; s < S; ++s) {
+            const float* ps = pp + (size_t)s * (NACC + 3) * cap + p;
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) {
-                const double hi = pp[((size_t)(s * NACC + k) * 2 + 0) * cap + p];
-                const double lo = pp[((size_t)(s * NACC + k) * 2 + 1) * cap + p];
-                A[k] += hi - lo;
-            }
+            for (int k = 0; k < NACC; ++k) A[k] += (double)ps[(size_t)k * cap];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) A[A_NX + k] -= (double)ps[(size_t)(NACC + k) * cap];
+        }
     }
 }
 
