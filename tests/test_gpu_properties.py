@@ -1,0 +1,217 @@
+"""Size-independent properties of the HIP path at full size, the
+FP64-vs-mixed criteria of the two benchmark workloads, and the sticky9
+host end to end."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import GOLDEN, ROOT, load_fixture, rel_err
+from pn_amd import lib as pn
+
+pytestmark = pytest.mark.gpu
+
+STICKY9 = os.path.join(ROOT, "pn_amd", "sticky9")
+C3 = dict(eps=1e-4, c=50.0, dt=2.0**-12)
+
+
+@pytest.fixture(scope="module")
+def big():
+    x, v, m = pn.ic_plummer(131072, 3, a=1.0, m_bh=0.05)
+    ctx = pn.Context(len(m))
+    yield ctx, x, v, m
+    ctx.close()
+
+
+def force_at(ctx, x, v, m, mode, eps, c, dt=2.0**-12, tuning=(0, 0, -1)):
+    ctx.set_tuning(*tuning)
+    ctx.set_params(eps, c, dt, mode)
+    ctx.upload(x, v, m)
+    return ctx.force()
+
+
+@pytest.mark.parametrize("mode", [pn.FP64, pn.MIXED])
+def test_determinism_full_size(big, mode):
+    ctx, x, v, m = big
+    a1 = force_at(ctx, x, v, m, mode, C3["eps"], C3["c"])
+    a2 = ctx.force()
+    assert np.array_equal(a1, a2)
+
+
+def test_graph_replay_equals_eager(big):
+    """pn1_step replays a captured hipGraph; with profiling on it launches
+    eagerly.  Both must give the same bits, replay after replay."""
+    ctx, x, v, m = big
+    ctx.set_tuning(0, 0, -1)
+    ctx.set_params(C3["eps"], C3["c"], C3["dt"], pn.MIXED)
+    ctx.upload(x, v, m)
+    ctx.profile_enable(True)
+    ctx.step(2)
+    xe, ve = ctx.download()
+    ctx.profile_enable(False)
+    ctx.upload(x, v, m)
+    ctx.step(1)
+    ctx.sync()
+    ctx.step(1)
+    xg, vg = ctx.download()
+    assert np.array_equal(xe, xg) and np.array_equal(ve, vg)
+
+
+def test_tuning_invariance(big):
+    ctx, x, v, m = big
+    ref = force_at(ctx, x, v, m, pn.FP64, C3["eps"], C3["c"], tuning=(64, 0, 0))
+    same = force_at(ctx, x, v, m, pn.FP64, C3["eps"], C3["c"], tuning=(64, 0, 1))
+    assert np.array_equal(ref, same)          # same order of operations
+    other = force_at(ctx, x, v, m, pn.FP64, C3["eps"], C3["c"], tuning=(16, 0, 0))
+    assert rel_err(other, ref).max() < 1e-13  # another split of the j sum
+    # the two mixed loop forms fold their Kahan sums after slightly different
+    # pair counts, so they agree to FP32 accuracy rather than bitwise
+    m1 = force_at(ctx, x, v, m, pn.MIXED, C3["eps"], C3["c"], tuning=(64, 16, 1))
+    m2 = force_at(ctx, x, v, m, pn.MIXED, C3["eps"], C3["c"], tuning=(64, 16, 0))
+    assert rel_err(m1, m2).max() < 1e-6
+
+
+def test_mixed_accuracy_at_headline_workload(big):
+    """N = 131072 Plummer + BH 0.05, c = 50, eps = 1e-4: the mixed form
+    against the FP64 form, per-particle relative error <= 1e-6."""
+    ctx, x, v, m = big
+    a64 = force_at(ctx, x, v, m, pn.FP64, C3["eps"], C3["c"])
+    a32 = force_at(ctx, x, v, m, pn.MIXED, C3["eps"], C3["c"])
+    err = rel_err(a32, a64)
+    print(f"C3 mixed vs fp64: p99.9 {np.quantile(err[1:], 0.999):.2e} max {err[1:].max():.2e} "
+          f"BH {err[0]:.2e}")
+    assert np.quantile(err[1:], 0.999) <= 1e-6
+    assert err[1:].max() <= 1e-6
+    assert err[0] <= 1e-8
+
+
+def test_newtonian_momentum_full_size(big):
+    ctx, x, v, m = big
+    a = force_at(ctx, x, v, m, pn.FP64, C3["eps"], 1e30)
+    p = (m[:, None] * a).sum(0)
+    assert np.abs(p).max() < 1e-12 * (m[:, None] * np.abs(a)).sum()
+
+
+def test_rotation_covariance_gpu():
+    x, v, m = pn.ic_uniform_sphere(4096, 11, R=1.0, Q=0.5, m_bh=0.01, eps=1e-3)
+    q, _ = np.linalg.qr(np.random.default_rng(4).normal(size=(3, 3)))
+    with pn.Context(len(m)) as ctx:
+        a = force_at(ctx, x, v, m, pn.FP64, 1e-3, 100.0)
+        ar = force_at(ctx, x @ q.T, v @ q.T, m, pn.FP64, 1e-3, 100.0)
+    assert rel_err(ar, a @ q.T).max() < 1e-12
+
+
+def test_energy_vs_oracle_n4097():
+    x, v, m = pn.ic_plummer(4096, 12, a=1.0, m_bh=0.02)
+    with pn.Context(len(m)) as ctx:
+        ctx.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+        ctx.upload(x, v, m)
+        e = ctx.energy()
+    eo = ob.orc_energy_terms(x, v, m, 1e-3, 100.0)
+    scale = (m * m).sum() / 1e-3
+    assert np.all(np.abs(e - eo) <= 1.6e-14 * scale + 1e-13 * np.abs(eo).max())
+
+
+def test_c2_energy_drift_and_accuracy():
+    """N = 10240 uniform sphere + BH 0.01, c = 100, eps = 1e-3, dt = 2^-10,
+    1000 steps: the mixed form's accelerations within 1e-6 of FP64, and its
+    relative energy drift within 2x of the FP64 run's."""
+    x, v, m = pn.ic_uniform_sphere(10240, 2, R=1.0, Q=0.5, m_bh=0.01, eps=1e-3)
+    drift = {}
+    with pn.Context(len(m)) as ctx:
+        a64 = force_at(ctx, x, v, m, pn.FP64, 1e-3, 100.0, 2.0**-10)
+        a32 = force_at(ctx, x, v, m, pn.MIXED, 1e-3, 100.0, 2.0**-10)
+        err = rel_err(a32, a64)
+        for mode in (pn.FP64, pn.MIXED):
+            ctx.set_params(1e-3, 100.0, 2.0**-10, mode)
+            ctx.upload(x, v, m)
+            e0 = ctx.energy()[4]
+            ctx.step(1000)
+            e1 = ctx.energy()[4]
+            drift[mode] = (e1 - e0) / abs(e0)
+    print(f"C2: mixed vs fp64 p99.9 {np.quantile(err[1:], 0.999):.2e} max {err[1:].max():.2e}; "
+          f"1000-step drift fp64 {drift[pn.FP64]:.3e} mixed {drift[pn.MIXED]:.3e}")
+    assert np.quantile(err[1:], 0.999) <= 1e-6 and err[1:].max() <= 1e-6
+    assert abs(drift[pn.MIXED]) <= 2.0 * abs(drift[pn.FP64])
+
+
+# ---------------------------------------------------------------- sticky9 host
+def run_sticky9(tmp_path, tag, *extra):
+    out = tmp_path / f"{tag}.dat"
+    r = subprocess.run([STICKY9, "--in", os.path.join(GOLDEN, "init10k_head.dat"), "--n", "65",
+                        "--dt", str(2.0**-10), "--deouttime", str(2.0**-9), "--steps", "6",
+                        "--out", str(out), *extra], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    snaps = np.array([[float(t) for t in ln.split()] for ln in out.read_text().splitlines()])
+    return r.stdout, snaps.reshape(-1, 2 * 65, 3)
+
+
+def totals(stdout):
+    return [float(t) for t in re.findall(r"total=\s*(\S+)", stdout)]
+
+
+def test_sticky9_end_to_end(tmp_path):
+    """The reference's driver restated on the drop-in library: legacy loop
+    (host stage arithmetic around force()) and resident loop agree, and both
+    follow the oracle's trajectory and energies (.dat head, LS9 masses)."""
+    out_l, snap_l = run_sticky9(tmp_path, "legacy", "--legacy")
+    out_r, snap_r = run_sticky9(tmp_path, "resident")
+    assert out_l.splitlines()[:2] == ["Init ", "start "] and out_l.endswith("0.005859 \n")
+    assert snap_l.shape == (5, 130, 3) and snap_r.shape == snap_l.shape  # t0, 3 outputs, end
+    assert np.abs(snap_l - snap_r).max() <= 1e-6
+    d = load_fixture("dat_n65")
+    x, v, m = d["x"], d["v"], d["m"]
+    want, e_or = [np.vstack([x, v])], []
+    E0 = ob.orc_energy_terms(x, v, m, 1e-3, 1.0)[4]
+    for k in range(3):
+        x, v = ob.orc_q1_steps(x, v, m, 1e-3, 1.0, 2.0**-10, 2)
+        want.append(np.vstack([x, v]))
+        e_or.append(ob.orc_energy_terms(x, v, m, 1e-3, 1.0)[4])
+    want.append(want[-1])
+    assert np.abs(snap_r - np.array(want)).max() <= 1.5e-6       # printed with %lf
+    for out in (out_l, out_r):
+        t = totals(out)
+        assert len(t) == 4
+        assert abs(t[0] - E0) <= 1e-12 * abs(E0)
+        assert np.allclose(t[1:], e_or, rtol=1e-11, atol=0)
+
+
+def test_sticky9_checkpoint_resume(tmp_path):
+    ck = tmp_path / "a.ckpt"
+    run_sticky9(tmp_path, "first", "--checkpoint", str(ck))
+    run_sticky9(tmp_path, "second", "--resume", str(ck), "--checkpoint", str(tmp_path / "b.ckpt"))
+    r = subprocess.run([STICKY9, "--in", os.path.join(GOLDEN, "init10k_head.dat"), "--n", "65",
+                        "--dt", str(2.0**-10), "--deouttime", str(2.0**-9), "--steps", "12",
+                        "--out", str(tmp_path / "straight.dat"), "--checkpoint",
+                        str(tmp_path / "c.ckpt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    b, c = pn.read_checkpoint(str(tmp_path / "b.ckpt")), pn.read_checkpoint(str(tmp_path / "c.ckpt"))
+    assert np.array_equal(b["x"], c["x"]) and np.array_equal(b["v"], c["v"])
+    assert b["step"] == c["step"] == 12 and b["t"] == c["t"]
+
+
+def test_legacy_energy_text(tmp_path):
+    """energy()/initial_energy() print the reference's text; the numbers
+    agree with the reference's printed ones."""
+    d = load_fixture("c1_n1025")
+    code = (
+        "import numpy as np, ctypes; from pn_amd import lib as pn\n"
+        f"z = np.load({os.path.join(GOLDEN, 'c1_n1025.npz')!r})\n"
+        "L = pn.load(); x, v, m = z['x'], z['v'], z['m']; e = ctypes.c_double()\n"
+        "L.initial_energy(pn.ptr(x), pn.ptr(v), pn.ptr(m), len(m), ctypes.byref(e), 1e-3)\n"
+        "L.energy(0.25, pn.ptr(x), pn.ptr(v), pn.ptr(m), len(m), float(z['E0_c1']) * (1 + 1e-6), 1e-3)\n")
+    r = subprocess.run(["python", "-c", code], capture_output=True, text=True, cwd=ROOT,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    ref = str(d["init_txt_c1"]) + str(d["energy_txt_c1"])
+    pat = r"[-+]?\d+\.\d+e[-+]\d+"
+    mine, theirs = re.findall(pat, r.stdout), re.findall(pat, ref)
+    assert len(mine) == len(theirs) == 11
+    assert re.sub(pat, "#", r.stdout) == re.sub(pat, "#", ref)   # same text around the numbers
+    for a, b in zip(mine, theirs):
+        digits = len(b.split("e")[0].split(".")[1])             # %22.15e fields vs %e fields
+        tol = 1e-12 if digits == 15 else 2e-6
+        assert abs(float(a) - float(b)) <= tol * abs(float(b)), (a, b)
