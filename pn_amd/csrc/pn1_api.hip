@@ -111,11 +111,12 @@ void plan(pn1_ctx* c)
     int S = c->jsplit_req;
     if (S <= 0) {
         // Many short rounds of workgroups: the last, partly filled round of
-        // a grid costs little when there are tens of rounds (measured at
-        // n = 131073 on MI355X: split 32 is 1.6x faster than split 4 for the
-        // mixed kernel).  Chunks stay >= 256 records.
+        // a grid costs little when there are tens of rounds (pn_amd/tune.py
+        // on MI355X: at n = 131073 split 64 is 1.6x faster than split 4 for
+        // the mixed kernel; at n = 10241 split 128 is 1.2x faster than 32).
+        // Chunks stay >= 64 records.
         S = 1;
-        while (S < 64 && (long)S * c->nIblk < 32768 && (nst + 2 * S - 1) / (2 * S) >= 256) S *= 2;
+        while (S < 128 && (long)S * c->nIblk < 32768 && (nst + 2 * S - 1) / (2 * S) >= 64) S *= 2;
     }
     if (nst > 0 && S > nst) S = nst;
     if (S < 1) S = 1;
