@@ -124,6 +124,50 @@ int  pn1_profile_read(pn1_ctx *ctx, int *launches, double *total_ms);
 int  pn1_util_copy(pn1_ctx *ctx, long long bytes, int reps);
 
 /* ====================================================================== */
+/* 2b. Ensemble API: many independent clusters (each a BH + its stars,    */
+/*     as one run of Limit-sticky9.c) advanced together on one device.    */
+/*     No counterpart in the reference, which runs one cluster per        */
+/*     process; every cluster's physics is the context API's.             */
+/* ====================================================================== */
+/* Cluster c is particles [offsets[c], offsets[c+1]) of the AoS arrays;
+ * its first particle is its BH.  All clusters share eps, c, dt and mode.
+ * One force evaluation of the ensemble is one persistent pair-kernel
+ * launch: workgroups take (cluster, 256-star block, j chunk) items from
+ * a device counter, largest first; each item writes its own partial sums,
+ * so results do not depend on the order items run in. */
+typedef struct pn1_ens pn1_ens;
+
+/* Up to `capacity` particles in total in up to `max_clusters` clusters. */
+int  pn1_ens_create(pn1_ens **out, int capacity, int max_clusters,
+                    int device);
+void pn1_ens_destroy(pn1_ens *ens);
+int  pn1_ens_set_params(pn1_ens *ens, double eps, double c, double dt,
+                        int mode);
+/* offsets[0] = 0, offsets[nclusters] = total particle count; every
+ * cluster is checked as pn1_upload checks one state. */
+int  pn1_ens_upload(pn1_ens *ens, int nclusters, const int *offsets,
+                    const double *x, const double *v, const double *m);
+int  pn1_ens_download(pn1_ens *ens, double *x, double *v);
+/* Accelerations of every particle, AoS [n_total][3]. */
+int  pn1_ens_force(pn1_ens *ens, double *a);
+/* Per-cluster energies: out[5 c + k] = {kin, pot, pot_pn, pot_pn2, total}
+ * (the fields of pn1_energy_t) of cluster c. */
+int  pn1_ens_energy(pn1_ens *ens, double *out);
+/* `steps` steps of every cluster (one hipGraph per step, replayed). */
+int  pn1_ens_step(pn1_ens *ens, int steps);
+int  pn1_ens_sync(pn1_ens *ens);
+const char *pn1_ens_last_error(const pn1_ens *ens);
+/* Work-item shape: j chunk length `chunk` (records per item, 0 = 512;
+ * applies from the next upload), mixed compensation interval `tile` and
+ * kernel `variant` as pn1_set_tuning. */
+int  pn1_ens_set_tuning(pn1_ens *ens, int chunk, int tile, int variant);
+int  pn1_ens_get_tuning(const pn1_ens *ens, int *chunk, int *tile,
+                        int *variant);
+/* Pair-kernel timing, as pn1_profile_enable / pn1_profile_read. */
+int  pn1_ens_profile_enable(pn1_ens *ens, int on);
+int  pn1_ens_profile_read(pn1_ens *ens, int *launches, double *total_ms);
+
+/* ====================================================================== */
 /* 3. Host-side helpers (no device needed).                               */
 /* ====================================================================== */
 /* Validates a state as the context API does; writes a reason to msg. */

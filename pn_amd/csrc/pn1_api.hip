@@ -13,6 +13,7 @@
 // pn1_energy and pn1_step only launch kernels and copy results.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "pn1.h"
+#include "pn1_ens_kernels.hpp"
 #include "pn1_kernels.hpp"
 
 using namespace pn1dev;
@@ -660,3 +662,5 @@ void initial_energy(double (*x)[3], double (*v)[3], double* m, int n, double* in
 }
 
 }  // extern "C"
+
+#include "pn1_ens.inc"

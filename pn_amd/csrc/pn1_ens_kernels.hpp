@@ -1,0 +1,334 @@
+// pn1_ens_kernels.hpp -- device code for ensembles of independent clusters.
+//
+// All clusters live in one state (SoA stride cap).  Cluster c occupies
+// particles [coff[c], coff[c + 1]); its BH is particle coff[c], its stars
+// follow.  One force evaluation of the whole ensemble is five launches:
+//
+//   k_ens_prep   per-star j-records and per-block separable sums; blocks
+//                come from a block table, each inside one cluster
+//   k_ens_gsum   one workgroup per cluster: that cluster's separable sums;
+//                also resets the work counter of the pair kernel
+//   k_ens_pair*  ONE persistent launch for all clusters: workgroups take
+//                (cluster, i-block, j-chunk) items from an atomic counter,
+//                items sorted by decreasing cost on the host; every item
+//                writes its partial sums to its own slot, so the result
+//                does not depend on which workgroup ran which item
+//   k_ens_fin    per star, as k_fin, with its cluster's sums and BH
+//   k_ens_bh     one workgroup per cluster: its BH
+//
+// The pair arithmetic is the single-cluster kernels' (body64 / body32).
+#pragma once
+#include "pn1_kernels.hpp"
+
+namespace pn1dev {
+
+// A pair work item: cluster, i-block (256 stars), j chunk.
+struct __align__(16) EItem {
+    int c, ib, s, pad;
+};
+
+// Star block of the O(n) kernels: cluster and first local star index.
+// Partial sums of cluster c start at slot poff[c] (a slot is one chunk of
+// one star; slots of c: nchunks(c) x its star count) times the number of
+// values a kernel keeps per slot (NACC, NACC + 3 or NACCE).
+struct __align__(8) EBlk {
+    int c, first;
+};
+
+// Ensemble launch constants: chunk length L (j-records per item).
+struct EConst {
+    int nitems, L, nclusters;
+};
+
+__device__ __forceinline__ int ens_nchunks(int nst, int L) { return (nst + L - 1) / L; }
+
+// The next work item for this workgroup, identical in every lane (and
+// known to the compiler as uniform).  Returns -1 when the queue is empty.
+__device__ __forceinline__ int ens_take(unsigned int* counter, int nitems)
+{
+    __shared__ int s_item;
+    __syncthreads();                         // every lane has read the last item
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(s_item);
+    return it < nitems ? it : -1;
+}
+
+template <int PP>
+__global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr,
+                                                  const double* __restrict__ st,
+                                                  double* __restrict__ part,
+                                                  const EItem* __restrict__ items,
+                                                  const int* __restrict__ coff,
+                                                  const long long* __restrict__ poff,
+                                                  unsigned int* __restrict__ counter, Phys P,
+                                                  EConst E)
+{
+    for (;;) {
+        const int it = ens_take(counter, E.nitems);
+        if (it < 0) break;
+        const EItem w = items[it];
+        const int off = coff[w.c], nst = coff[w.c + 1] - off - 1;
+        const int li = w.ib * WG + (int)threadIdx.x;          // local star index
+        const int i = off + 1 + (li < nst ? li : nst - 1);
+        const I64 I = load_i64(jr, st, P.cap, i, 0.0, P.eps2);
+        const int j0 = off + 1 + w.s * E.L, j1 = min(off + 1 + nst, j0 + E.L);
+        const int wb = off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
+        const int w0 = max(j0, wb), w1 = min(j1, wb + 64);
+        double a[NACC];
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+        if (w0 < w1) {
+            run64<false, PP>(jr, j0, w0, i, I, a, P.eps2);
+            run64<true, PP>(jr, w0, w1, i, I, a, P.eps2);
+            run64<false, PP>(jr, w1, j1, i, I, a, P.eps2);
+        } else {
+            run64<false, PP>(jr, j0, j1, i, I, a, P.eps2);
+        }
+        if (li < nst) {
+            double* pp = part + poff[w.c] * NACC + li;
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) pp[(size_t)(w.s * NACC + k) * nst] = a[k];
+        }
+    }
+}
+
+template <int PP>
+__global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr,
+                                                  const double* __restrict__ st,
+                                                  float* __restrict__ part,
+                                                  const EItem* __restrict__ items,
+                                                  const int* __restrict__ coff,
+                                                  const long long* __restrict__ poff,
+                                                  unsigned int* __restrict__ counter, Phys P,
+                                                  EConst E, int tile)
+{
+    for (;;) {
+        const int it = ens_take(counter, E.nitems);
+        if (it < 0) break;
+        const EItem w = items[it];
+        const int off = coff[w.c], nst = coff[w.c + 1] - off - 1;
+        const int li = w.ib * WG + (int)threadIdx.x;
+        const int i = off + 1 + (li < nst ? li : nst - 1);
+        const I32 I = load_i32(jr, st, P.cap, i, 0.0, P.eps2);
+        const int j0 = off + 1 + w.s * E.L, j1 = min(off + 1 + nst, j0 + E.L);
+        const int wb = off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
+        const int w0 = max(j0, wb), w1 = min(j1, wb + 64);
+        float a[NACC], sN[3], cN[3];
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sN[k] = cN[k] = 0.0f;
+        int cnt = 0;
+        if (w0 < w1) {
+            run32<false, PP>(jr, j0, w0, i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<true, PP>(jr, w0, w1, i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<false, PP>(jr, w1, j1, i, I, a, P.eps2f, sN, cN, tile, cnt);
+        } else {
+            run32<false, PP>(jr, j0, j1, i, I, a, P.eps2f, sN, cN, tile, cnt);
+        }
+        fold_newton(a, sN, cN);
+        if (li < nst) {
+            float* pp = part + poff[w.c] * (NACC + 3) + (size_t)w.s * (NACC + 3) * nst + li;
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) pp[(size_t)k * nst] = k <= A_NZ ? sN[k] : a[k];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pp[(size_t)(NACC + k) * nst] = cN[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
+                                                const double* __restrict__ m,
+                                                const double* __restrict__ cm0,
+                                                const int* __restrict__ coff,
+                                                const EBlk* __restrict__ blk, Phys P,
+                                                JRec64* __restrict__ j64,
+                                                JRec32* __restrict__ j32,
+                                                double* __restrict__ gblk)
+{
+    const EBlk b = blk[blockIdx.x];
+    const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
+    const int li = b.first + (int)threadIdx.x;
+    double g[NGS];
+#pragma unroll
+    for (int k = 0; k < NGS; ++k) g[k] = 0.0;
+    if (li < nst) prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, j32, g);
+    block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
+}
+
+// One workgroup per cluster: rows [cblk[c], cblk[c+1]) of `rows` (N wide).
+template <int N>
+__device__ __forceinline__ void cluster_rows_sum(const double* __restrict__ rows,
+                                                 const int* __restrict__ cblk, int c,
+                                                 double (&g)[N])
+{
+#pragma unroll
+    for (int k = 0; k < N; ++k) g[k] = 0.0;
+    for (int b = cblk[c] + (int)threadIdx.x; b < cblk[c + 1]; b += WG)
+#pragma unroll
+        for (int k = 0; k < N; ++k) g[k] += rows[(size_t)b * N + k];
+}
+
+__global__ __launch_bounds__(WG) void k_ens_gsum(const double* __restrict__ gblk,
+                                                const int* __restrict__ cblk,
+                                                double* __restrict__ gs,
+                                                unsigned int* __restrict__ counter)
+{
+    const int c = blockIdx.x;
+    if (c == 0 && threadIdx.x == 0) counter[0] = 0u;
+    double g[NGS];
+    cluster_rows_sum<NGS>(gblk, cblk, c, g);
+    block_sum<NGS>(g, gs + (size_t)c * NGS);
+}
+
+template <int MODE, int STAGE>
+__global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, double* Y,
+                                               double* H1, double* H2,
+                                               const double* __restrict__ m,
+                                               const double* __restrict__ cm0,
+                                               const int* __restrict__ coff,
+                                               const EBlk* __restrict__ blk,
+                                               const void* __restrict__ part,
+                                               const long long* __restrict__ poff,
+                                               const double* __restrict__ gs,
+                                               double* __restrict__ acc,
+                                               double* __restrict__ bhblk, Phys P, EConst E)
+{
+    const EBlk b = blk[blockIdx.x];
+    const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
+    const int li = b.first + (int)threadIdx.x;
+    double cb[3] = {0.0, 0.0, 0.0};
+    if (li < nst) {
+        double S_[NACC];
+        const void* base = MODE == 0 ? (const void*)((const double*)part + poff[b.c] * NACC)
+                                     : (const void*)((const float*)part + poff[b.c] * (NACC + 3));
+        gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li, S_);
+        fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, off + 1 + li, off, S_,
+                        gs + (size_t)b.c * NGS, fin_consts(P, cm0[b.c]), acc, cb);
+    }
+    block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(WG) void k_ens_bh(const double* __restrict__ bhblk,
+                                              const int* __restrict__ cblk,
+                                              const int* __restrict__ coff,
+                                              const double* __restrict__ gs, double* X,
+                                              double* Y, double* H1, double* H2,
+                                              double* __restrict__ acc, Phys P)
+{
+    const int c = blockIdx.x;
+    double s[3];
+    cluster_rows_sum<3>(bhblk, cblk, c, s);
+    __shared__ double tot[3];
+    block_sum<3>(s, tot);
+    if (threadIdx.x == 0) {
+        const double t[3] = {tot[0], tot[1], tot[2]};
+        bh_finish<STAGE>(t, gs + (size_t)c * NGS, coff[c], P.cap, P.lam, P.dt, X, Y, H1, H2,
+                         acc);
+    }
+}
+
+// ---------------------------------------------------------------------
+// ensemble energies (FP64)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ jr,
+                                                   const double* __restrict__ st,
+                                                   double* __restrict__ part,
+                                                   const EItem* __restrict__ items,
+                                                   const int* __restrict__ coff,
+                                                   const long long* __restrict__ poff,
+                                                   unsigned int* __restrict__ counter, Phys P,
+                                                   EConst E)
+{
+    for (;;) {
+        const int it = ens_take(counter, E.nitems);
+        if (it < 0) break;
+        const EItem w = items[it];
+        const int off = coff[w.c], nst = coff[w.c + 1] - off - 1;
+        const int li = w.ib * WG + (int)threadIdx.x;
+        const int i = off + 1 + (li < nst ? li : nst - 1);
+        IE64 I;
+        I.x = jr[i].x; I.y = jr[i].y; I.z = jr[i].z;
+        I.ux = st[3 * P.cap + i] / PN1_SQ15;
+        I.uy = st[4 * P.cap + i] / PN1_SQ15;
+        I.uz = st[5 * P.cap + i] / PN1_SQ15;
+        const int j0 = off + 1 + w.s * E.L, j1 = min(off + 1 + nst, j0 + E.L);
+        const int wb = off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
+        const int w0 = max(j0, wb), w1 = min(j1, wb + 64);
+        double a[NACCE];
+#pragma unroll
+        for (int k = 0; k < NACCE; ++k) a[k] = 0.0;
+        if (w0 < w1) {
+            erun64<false>(jr, j0, w0, i, I, a, P.eps2);
+            erun64<true>(jr, w0, w1, i, I, a, P.eps2);
+            erun64<false>(jr, w1, j1, i, I, a, P.eps2);
+        } else {
+            erun64<false>(jr, j0, j1, i, I, a, P.eps2);
+        }
+        if (li < nst) {
+            double* pp = part + poff[w.c] * NACCE + li;
+#pragma unroll
+            for (int k = 0; k < NACCE; ++k) pp[(size_t)(w.s * NACCE + k) * nst] = a[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_ens_efin(const double* __restrict__ st,
+                                                const double* __restrict__ m,
+                                                const double* __restrict__ cm0,
+                                                const int* __restrict__ coff,
+                                                const EBlk* __restrict__ blk,
+                                                const double* __restrict__ part,
+                                                const long long* __restrict__ poff,
+                                                const double* __restrict__ gs,
+                                                double* __restrict__ eblk, Phys P, EConst E)
+{
+    const EBlk b = blk[blockIdx.x];
+    const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
+    const int li = b.first + (int)threadIdx.x;
+    double t[NEB] = {0.0, 0.0, 0.0, 0.0};
+    if (li < nst) {
+        const double* pp = part + poff[b.c] * NACCE + li;
+        double A[NACCE] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const int S = ens_nchunks(nst, E.L);
+        for (int s = 0; s < S; ++s)
+#pragma unroll
+            for (int k = 0; k < NACCE; ++k) A[k] += pp[(size_t)(s * NACCE + k) * nst];
+        efin_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, A,
+                  gs[(size_t)b.c * NGS + G_G1], t);
+    }
+    block_sum<NEB>(t, eblk + (size_t)blockIdx.x * NEB);
+}
+
+// out[c] = {kin, pot, pot_pn/c^2, pot_pn2/c^2, total} of cluster c
+__global__ __launch_bounds__(WG) void k_ens_ered(const double* __restrict__ eblk,
+                                                const int* __restrict__ cblk,
+                                                const int* __restrict__ coff,
+                                                const double* __restrict__ st,
+                                                const double* __restrict__ m,
+                                                const double* __restrict__ gs,
+                                                double* __restrict__ out, Phys P)
+{
+    const int c = blockIdx.x;
+    double s[NEB];
+    cluster_rows_sum<NEB>(eblk, cblk, c, s);
+    __shared__ double tot[NEB];
+    block_sum<NEB>(s, tot);
+    if (threadIdx.x == 0) {
+        const int b0 = coff[c], cap = P.cap;
+        const double v0x = st[3 * cap + b0], v0y = st[4 * cap + b0], v0z = st[5 * cap + b0];
+        const double kin = 0.5 * (m[b0] * (v0x * v0x + v0y * v0y + v0z * v0z) + tot[0]);
+        const double pot = tot[1] - m[b0] * gs[(size_t)c * NGS + G_G1];
+        const double pn = P.lam * tot[2], pn2 = P.lam * tot[3];
+        double* o = out + (size_t)c * 5;
+        o[0] = kin;
+        o[1] = pot;
+        o[2] = pn;
+        o[3] = pn2;
+        o[4] = kin + pot + pn + pn2;
+    }
+}
+
+}  // namespace pn1dev

@@ -182,53 +182,62 @@ __device__ __forceinline__ void stage_update(int p, int cap, const double (&a)[3
 // ---------------------------------------------------------------------
 // k_prep: per-star BH geometry + separable sums
 // ---------------------------------------------------------------------
+// Star p (SoA stride cap) of a cluster whose BH sits at index b0 and has
+// mass m0: writes its j-records and adds its terms of the separable sums
+// to g.
+__device__ __forceinline__ void prep_star(const double* __restrict__ st,
+                                          const double* __restrict__ m, int cap, int p,
+                                          int b0, double m0, double eps2,
+                                          JRec64* __restrict__ j64, JRec32* __restrict__ j32,
+                                          double (&g)[NGS])
+{
+#pragma clang fp contract(off)
+    const double x = st[p], y = st[cap + p], z = st[2 * cap + p];
+    const double vx = st[3 * cap + p], vy = st[4 * cap + p], vz = st[5 * cap + p];
+    const double mp = m[p];
+    const double dx = x - st[b0], dy = y - st[cap + b0], dz = z - st[2 * cap + b0];
+    const double D2 = dx * dx + dy * dy + dz * dz;
+    const double e = 1.0 / sqrt(D2 + eps2);     // as Limit-sticky9.c:106-108
+    const double E3 = e * e * e;
+    const double q = mp * E3;
+    const double V2 = vx * vx + vy * vy + vz * vz;
+    JRec64 r;
+    r.x = x; r.y = y; r.z = z;
+    r.sx = PN1_SQ15 * vx; r.sy = PN1_SQ15 * vy; r.sz = PN1_SQ15 * vz;
+    r.m = mp; r.D2 = D2;
+    r.A = 1.25 * m0 * e - 2.0 * V2; r.B = 0.25 * m0 * E3;
+    r.Q = 3.5 * m0 * q; r.MB = mp * r.B;
+    r.dx = dx; r.dy = dy; r.dz = dz; r.e = e;
+    j64[p] = r;
+    if (j32) {
+        JRec32 h;
+        h.xh = (float)x; h.yh = (float)y; h.zh = (float)z;
+        h.xl = (float)(x - (double)h.xh);
+        h.yl = (float)(y - (double)h.yh);
+        h.zl = (float)(z - (double)h.zh);
+        h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
+        h.m = (float)mp; h.D2 = (float)D2;
+        h.A = (float)r.A; h.B = (float)r.B; h.Q = (float)r.Q; h.MB = (float)r.MB;
+        h.dx = (float)dx; h.dy = (float)dy; h.dz = (float)dz; h.e = (float)e;
+        h.pad = 0.0f;
+        j32[p] = h;
+    }
+    g[G_G1] = mp * e;
+    g[G_M1X] = q * dx; g[G_M1Y] = q * dy; g[G_M1Z] = q * dz;
+    g[G_PX] = mp * vx; g[G_PY] = mp * vy; g[G_PZ] = mp * vz;
+}
+
 __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
                                             const double* __restrict__ m, Phys P,
                                             JRec64* __restrict__ j64,
                                             JRec32* __restrict__ j32,
                                             double* __restrict__ gblk)
 {
-#pragma clang fp contract(off)
     const int p = blockIdx.x * WG + threadIdx.x + 1;
-    const int cap = P.cap;
     double g[NGS];
 #pragma unroll
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
-    if (p < P.n) {
-        const double x = st[p], y = st[cap + p], z = st[2 * cap + p];
-        const double vx = st[3 * cap + p], vy = st[4 * cap + p], vz = st[5 * cap + p];
-        const double mp = m[p];
-        const double dx = x - st[0], dy = y - st[cap], dz = z - st[2 * cap];
-        const double D2 = dx * dx + dy * dy + dz * dz;
-        const double e = 1.0 / sqrt(D2 + P.eps2);     // as Limit-sticky9.c:106-108
-        const double E3 = e * e * e;
-        const double q = mp * E3;
-        const double V2 = vx * vx + vy * vy + vz * vz;
-        JRec64 r;
-        r.x = x; r.y = y; r.z = z;
-        r.sx = PN1_SQ15 * vx; r.sy = PN1_SQ15 * vy; r.sz = PN1_SQ15 * vz;
-        r.m = mp; r.D2 = D2;
-        r.A = 1.25 * P.m0 * e - 2.0 * V2; r.B = 0.25 * P.m0 * E3;
-        r.Q = 3.5 * P.m0 * q; r.MB = mp * r.B;
-        r.dx = dx; r.dy = dy; r.dz = dz; r.e = e;
-        j64[p] = r;
-        if (j32) {
-            JRec32 h;
-            h.xh = (float)x; h.yh = (float)y; h.zh = (float)z;
-            h.xl = (float)(x - (double)h.xh);
-            h.yl = (float)(y - (double)h.yh);
-            h.zl = (float)(z - (double)h.zh);
-            h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
-            h.m = (float)mp; h.D2 = (float)D2;
-            h.A = (float)r.A; h.B = (float)r.B; h.Q = (float)r.Q; h.MB = (float)r.MB;
-            h.dx = (float)dx; h.dy = (float)dy; h.dz = (float)dz; h.e = (float)e;
-            h.pad = 0.0f;
-            j32[p] = h;
-        }
-        g[G_G1] = mp * e;
-        g[G_M1X] = q * dx; g[G_M1Y] = q * dy; g[G_M1Z] = q * dz;
-        g[G_PX] = mp * vx; g[G_PY] = mp * vy; g[G_PZ] = mp * vz;
-    }
+    if (p < P.n) prep_star(st, m, P.cap, p, 0, P.m0, P.eps2, j64, j32, g);
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
 }
 
@@ -569,9 +578,13 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
 // ---------------------------------------------------------------------
 // k_fin: star accelerations, stage update, BH partials
 // ---------------------------------------------------------------------
+// Sum of the S chunk partials of one star.  `part` points at the
+// cluster's partial area, `stride` is its per-sum stride and `idx` the
+// star's index in it.  FP64: [S][NACC][stride]; mixed: [S][NACC + 3][stride]
+// floats, the last 3 rows holding the Newtonian Kahan compensations.
 template <int MODE>
-__device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S, int cap, int p,
-                                           double (&A)[NACC])
+__device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S, size_t stride,
+                                           int idx, double (&A)[NACC])
 {
 #pragma unroll
     for (int k = 0; k < NACC; ++k) A[k] = 0.0;
@@ -579,17 +592,113 @@ __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S,
         const double* pp = (const double*)part;
         for (int s = 0; s < S; ++s)
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) A[k] += pp[(size_t)(s * NACC + k) * cap + p];
+            for (int k = 0; k < NACC; ++k) A[k] += pp[(size_t)(s * NACC + k) * stride + idx];
     } else {
         const float* pp = (const float*)part;
         for (int s = 0; s < S; ++s) {
-            const float* ps = pp + (size_t)s * (NACC + 3) * cap + p;
+            const float* ps = pp + (size_t)s * (NACC + 3) * stride + idx;
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) A[k] += (double)ps[(size_t)k * cap];
+            for (int k = 0; k < NACC; ++k) A[k] += (double)ps[(size_t)k * stride];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) A[A_NX + k] -= (double)ps[(size_t)(NACC + k) * cap];
+            for (int k = 0; k < 3; ++k) A[A_NX + k] -= (double)ps[(size_t)(NACC + k) * stride];
         }
     }
+}
+
+// Run constants shared by the O(n) passes of one cluster.
+struct FinC {
+    double m0, eps, eps2, ws, lam, dt;
+};
+
+// Star p (BH of its cluster at index b0): its acceleration from its pair
+// sums S_ and the cluster's separable sums gs, the stage update, and its
+// share cb of the BH's 1PN acceleration (a1 + a1c).
+template <int STAGE>
+__device__ __forceinline__ void fin_star(const double* in, double* X, double* Y, double* H1,
+                                         double* H2, const double* __restrict__ m, int cap,
+                                         int p, int b0, const double (&S_)[NACC],
+                                         const double* __restrict__ gs, const FinC& C,
+                                         double* __restrict__ acc, double (&cb)[3])
+{
+    const double m0 = C.m0, mp = m[p], eps2 = C.eps2;
+    const double x[3] = {in[p], in[cap + p], in[2 * cap + p]};
+    const double v[3] = {in[3 * cap + p], in[4 * cap + p], in[5 * cap + p]};
+    const double x0[3] = {in[b0], in[cap + b0], in[2 * cap + b0]};
+    double d[3], D2, e;
+    {
+#pragma clang fp contract(off)
+        for (int k = 0; k < 3; ++k) d[k] = x[k] - x0[k];
+        D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        e = 1.0 / sqrt(D2 + eps2);
+    }
+    const double E3 = e * e * e, q = mp * E3;
+    const double G1 = gs[G_G1];
+    const double M1[3] = {gs[G_M1X], gs[G_M1Y], gs[G_M1Z]};
+    const double Pm[3] = {gs[G_PX], gs[G_PY], gs[G_PZ]};
+    const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double vd = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
+
+    // ---- star i: Newton + B + S + sum over j != i of the ac pair term
+    const double Pp[3] = {Pm[0] - mp * v[0], Pm[1] - mp * v[1], Pm[2] - mp * v[2]};
+    const double vP = v[0] * Pp[0] + v[1] * Pp[1] + v[2] * Pp[2];
+    const double Pd = Pp[0] * d[0] + Pp[1] * d[1] + Pp[2] * d[2];
+    const double c1 = -m0 * E3 * (4.0 * S_[A_PHI] + 1.25 * (G1 - mp * e)) + E3 * S_[A_ZE];
+    const double cB = 4.0 * m0 * e - V2;   // also the i-only part of the K_r bracket
+    const double cS = 5.0 * m0 * mp * e * e * e * e - 4.0 * mp * E3 * V2;
+    double a[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double newton = S_[A_NX + k] - m0 * E3 * d[k];
+        const double B = -m0 * E3 * (d[k] * cB + 4.0 * vd * v[k]);
+        const double Sg = cS * d[k] + 7.0 * mp * E3 * vd * v[k];
+        const double t4 = E3 * (4.0 * vP * d[k] - 3.0 * Pd * v[k] - 4.0 * vd * Pp[k]);
+        const double ru = S_[A_UX + k] / PN1_SQ15 - v[k] * S_[A_KS];
+        const double t3 = S_[A_TX + k] - 3.5 * m0 * e * (M1[k] - q * d[k]);
+        const double rr = S_[A_RX + k] + cB * S_[A_NX + k];
+        const double pn = B + Sg + rr + ru + c1 * d[k] + t3 + t4;
+        a[k] = newton + C.lam * pn;
+        acc[(size_t)k * cap + p] = a[k];
+    }
+
+    // ---- this star's share of the BH's 1PN acceleration: a1 + a1c
+    // the j = i bracket of the a1c pair term in the reference's operation
+    // order (Limit-sticky9.c:139-140 at |r|^2 = 0)
+    double self_br;
+    {
+#pragma clang fp contract(off)
+        const double ws = C.ws;
+        self_br = 4.0 * e + 1.25 * ws - 0.25 * D2 / eps2 * ws + 0.25 * D2 / eps2 * ws;
+    }
+    const double vP0 = v[0] * Pm[0] + v[1] * Pm[1] + v[2] * Pm[2];
+    const double Pd0 = Pm[0] * d[0] + Pm[1] * d[1] + Pm[2] * d[2];
+    const double mE3 = mp * E3;
+    const double cA1 = mE3 * (5.0 * m0 * e - 2.0 * V2 + 1.5 * vd * vd / D2);
+    const double cC0 = 4.0 * mp * mp * e * e * e * e;
+    const double cC2 = mE3 * (4.0 * (G1 - mp * e) + 1.25 * S_[A_PHI] + 0.25 * S_[A_XI]
+                              + mp * self_br);
+    const double cQ2 = mp * mp * E3 * (4.0 * e + 1.25 * C.eps);
+    const double vm = mE3 / m0, vm2 = mp * mp * E3 / m0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double A1 = cA1 * d[k] + 3.0 * mE3 * vd * v[k];
+        const double C2 = cC2 * d[k] - 3.5 * e * mp * S_[A_NX + k]
+                          - vm * (4.0 * vP0 * d[k] - 3.0 * vd * Pm[k] - 4.0 * Pd0 * v[k]);
+        const double Q2 = cQ2 * d[k] - vm2 * (4.0 * V2 * d[k] - 7.0 * vd * v[k]);
+        cb[k] = A1 + cC0 * d[k] + C2 + Q2;
+    }
+    stage_update<STAGE>(p, cap, a, X, Y, H1, H2, C.dt);
+}
+
+__device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
+{
+    FinC C;
+    C.m0 = m0;
+    C.eps = P.eps;
+    C.eps2 = P.eps2;
+    C.ws = P.ws;
+    C.lam = P.lam;
+    C.dt = P.dt;
+    return C;
 }
 
 template <int MODE, int STAGE>
@@ -601,83 +710,32 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
                                            double* __restrict__ bhblk, Phys P)
 {
     const int p = blockIdx.x * WG + threadIdx.x + 1;
-    const int cap = P.cap;
     double cb[3] = {0.0, 0.0, 0.0};
     if (p < P.n) {
-        const double m0 = P.m0, mp = m[p], eps2 = P.eps2;
-        const double x[3] = {in[p], in[cap + p], in[2 * cap + p]};
-        const double v[3] = {in[3 * cap + p], in[4 * cap + p], in[5 * cap + p]};
-        const double x0[3] = {in[0], in[cap], in[2 * cap]};
-        double d[3], D2, e;
-        {
-#pragma clang fp contract(off)
-            for (int k = 0; k < 3; ++k) d[k] = x[k] - x0[k];
-            D2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            e = 1.0 / sqrt(D2 + eps2);
-        }
-        const double E3 = e * e * e, q = mp * E3;
         double S_[NACC];
-        gather_acc<MODE>(part, S, cap, p, S_);
-        const double G1 = gs[G_G1];
-        const double M1[3] = {gs[G_M1X], gs[G_M1Y], gs[G_M1Z]};
-        const double Pm[3] = {gs[G_PX], gs[G_PY], gs[G_PZ]};
-        const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        const double vd = v[0] * d[0] + v[1] * d[1] + v[2] * d[2];
-
-        // ---- star i: Newton + B + S + sum over j != i of the ac pair term
-        const double Pp[3] = {Pm[0] - mp * v[0], Pm[1] - mp * v[1], Pm[2] - mp * v[2]};
-        const double vP = v[0] * Pp[0] + v[1] * Pp[1] + v[2] * Pp[2];
-        const double Pd = Pp[0] * d[0] + Pp[1] * d[1] + Pp[2] * d[2];
-        const double c1 = -m0 * E3 * (4.0 * S_[A_PHI] + 1.25 * (G1 - mp * e)) + E3 * S_[A_ZE];
-        const double cB = 4.0 * m0 * e - V2;   // also the i-only part of the K_r bracket
-        const double cS = 5.0 * m0 * mp * e * e * e * e - 4.0 * mp * E3 * V2;
-        double a[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double newton = S_[A_NX + k] - m0 * E3 * d[k];
-            const double B = -m0 * E3 * (d[k] * cB + 4.0 * vd * v[k]);
-            const double Sg = cS * d[k] + 7.0 * mp * E3 * vd * v[k];
-            const double t4 = E3 * (4.0 * vP * d[k] - 3.0 * Pd * v[k] - 4.0 * vd * Pp[k]);
-            const double ru = S_[A_UX + k] / PN1_SQ15 - v[k] * S_[A_KS];
-            const double t3 = S_[A_TX + k] - 3.5 * m0 * e * (M1[k] - q * d[k]);
-            const double rr = S_[A_RX + k] + cB * S_[A_NX + k];
-            const double pn = B + Sg + rr + ru + c1 * d[k] + t3 + t4;
-            a[k] = newton + P.lam * pn;
-            acc[(size_t)k * cap + p] = a[k];
-        }
-
-        // ---- this star's share of the BH's 1PN acceleration: a1 + a1c
-        // the j = i bracket of the a1c pair term in the reference's
-        // operation order (Limit-sticky9.c:139-140 at |r|^2 = 0)
-        double self_br;
-        {
-#pragma clang fp contract(off)
-            const double ws = P.ws;
-            self_br = 4.0 * e + 1.25 * ws - 0.25 * D2 / eps2 * ws + 0.25 * D2 / eps2 * ws;
-        }
-        const double vP0 = v[0] * Pm[0] + v[1] * Pm[1] + v[2] * Pm[2];
-        const double Pd0 = Pm[0] * d[0] + Pm[1] * d[1] + Pm[2] * d[2];
-        const double mE3 = mp * E3;
-        const double cA1 = mE3 * (5.0 * m0 * e - 2.0 * V2 + 1.5 * vd * vd / D2);
-        const double cC0 = 4.0 * mp * mp * e * e * e * e;
-        const double cC2 = mE3 * (4.0 * (G1 - mp * e) + 1.25 * S_[A_PHI] + 0.25 * S_[A_XI]
-                                  + mp * self_br);
-        const double cQ2 = mp * mp * E3 * (4.0 * e + 1.25 * P.eps);
-        const double vm = mE3 / m0, vm2 = mp * mp * E3 / m0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double A1 = cA1 * d[k] + 3.0 * mE3 * vd * v[k];
-            const double C2 = cC2 * d[k] - 3.5 * e * mp * S_[A_NX + k]
-                              - vm * (4.0 * vP0 * d[k] - 3.0 * vd * Pm[k] - 4.0 * Pd0 * v[k]);
-            const double Q2 = cQ2 * d[k] - vm2 * (4.0 * V2 * d[k] - 7.0 * vd * v[k]);
-            cb[k] = A1 + cC0 * d[k] + C2 + Q2;
-        }
-        stage_update<STAGE>(p, cap, a, X, Y, H1, H2, P.dt);
+        gather_acc<MODE>(part, S, (size_t)P.cap, p, S_);
+        fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb);
     }
     block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
 }
 
-// BH: Newton sum_j m_j e_j^3 d_j plus lam times the star shares.
+// BH at index b0: Newton sum_j m_j e_j^3 d_j plus lam times the star shares
+// tot; stage update.  Thread 0 only.
+template <int STAGE>
+__device__ __forceinline__ void bh_finish(const double (&tot)[3], const double* __restrict__ gs,
+                                          int b0, int cap, double lam, double dt, double* X,
+                                          double* Y, double* H1, double* H2,
+                                          double* __restrict__ acc)
+{
+    double a0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a0[k] = gs[G_M1X + k] + lam * tot[k];
+        acc[(size_t)k * cap + b0] = a0[k];
+    }
+    stage_update<STAGE>(b0, cap, a0, X, Y, H1, H2, dt);
+}
+
 template <int STAGE>
 __global__ __launch_bounds__(WG) void k_bh(const double* __restrict__ bhblk, int nblk,
                                           const double* __restrict__ gs, double* X, double* Y,
@@ -691,13 +749,8 @@ __global__ __launch_bounds__(WG) void k_bh(const double* __restrict__ bhblk, int
     __shared__ double tot[3];
     block_sum<3>(s, tot);
     if (threadIdx.x == 0) {
-        double a0[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            a0[k] = gs[G_M1X + k] + P.lam * tot[k];
-            acc[(size_t)k * P.cap] = a0[k];
-        }
-        stage_update<STAGE>(0, P.cap, a0, X, Y, H1, H2, P.dt);
+        const double t[3] = {tot[0], tot[1], tot[2]};
+        bh_finish<STAGE>(t, gs, 0, P.cap, P.lam, P.dt, X, Y, H1, H2, acc);
     }
 }
 
@@ -778,7 +831,36 @@ __global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
 }
 
 // Per-star energy terms of Limit-sticky9.c:452-550 with the self pair
-// excluded; per-block partials {sum m v^2, -1/2 sum m phi, pot_pn, pot_pn2}.
+// excluded, for star p of the cluster whose BH is at b0, from its energy
+// pair sums A: {m v^2, -1/2 m phi, pot_pn term, pot_pn2 term}.
+__device__ __forceinline__ void efin_star(const double* __restrict__ st,
+                                          const double* __restrict__ m, int cap, int p, int b0,
+                                          double m0, double eps2, const double (&A)[NACCE],
+                                          double G1, double (&t)[NEB])
+{
+    const double mp = m[p];
+    const double x[3] = {st[p], st[cap + p], st[2 * cap + p]};
+    const double v[3] = {st[3 * cap + p], st[4 * cap + p], st[5 * cap + p]};
+    const double v0[3] = {st[3 * cap + b0], st[4 * cap + b0], st[5 * cap + b0]};
+    double d[3], e;
+    {
+#pragma clang fp contract(off)
+        for (int k = 0; k < 3; ++k) d[k] = x[k] - st[(size_t)k * cap + b0];
+        e = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + eps2);
+    }
+    const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double v0v = v0[0] * v[0] + v0[1] * v[1] + v0[2] * v[2];
+    const double dv0 = d[0] * v0[0] + d[1] * v0[1] + d[2] * v0[2];
+    const double dv = d[0] * v[0] + d[1] * v[1] + d[2] * v[2];
+    const double vPhv = (v[0] * A[E_PVX] + v[1] * A[E_PVY] + v[2] * A[E_PVZ]) / PN1_SQ15;
+    t[0] = mp * V2;
+    t[1] = -0.5 * mp * A[E_PHI];
+    t[2] = 0.375 * mp * V2 * V2 + 1.5 * m0 * e * mp * V2 + 0.5 * m0 * m0 * e * e * mp
+           + 0.5 * m0 * e * mp * (3.0 * V2 - 7.0 * v0v - dv0 * dv * e * e);
+    t[3] = mp * (0.25 * (6.0 * V2 * A[E_PHI] - 7.0 * vPhv - A[E_GAM]) + m0 * e * A[E_PHI]
+                 + 0.5 * m0 * e * (G1 - mp * e));
+}
+
 __global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
                                             const double* __restrict__ m,
                                             const double* __restrict__ part, int S,
@@ -786,35 +868,13 @@ __global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
                                             double* __restrict__ eblk, Phys P)
 {
     const int p = blockIdx.x * WG + threadIdx.x + 1;
-    const int cap = P.cap;
     double t[NEB] = {0.0, 0.0, 0.0, 0.0};
     if (p < P.n) {
-        const double m0 = P.m0, mp = m[p];
-        const double x[3] = {st[p], st[cap + p], st[2 * cap + p]};
-        const double v[3] = {st[3 * cap + p], st[4 * cap + p], st[5 * cap + p]};
-        const double v0[3] = {st[3 * cap], st[4 * cap], st[5 * cap]};
-        double d[3], e;
-        {
-#pragma clang fp contract(off)
-            for (int k = 0; k < 3; ++k) d[k] = x[k] - st[(size_t)k * cap];
-            e = 1.0 / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + P.eps2);
-        }
         double A[NACCE] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int s = 0; s < S; ++s)
 #pragma unroll
-            for (int k = 0; k < NACCE; ++k) A[k] += part[(size_t)(s * NACCE + k) * cap + p];
-        const double V2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-        const double v0v = v0[0] * v[0] + v0[1] * v[1] + v0[2] * v[2];
-        const double dv0 = d[0] * v0[0] + d[1] * v0[1] + d[2] * v0[2];
-        const double dv = d[0] * v[0] + d[1] * v[1] + d[2] * v[2];
-        const double vPhv = (v[0] * A[E_PVX] + v[1] * A[E_PVY] + v[2] * A[E_PVZ]) / PN1_SQ15;
-        const double G1 = gs[G_G1];
-        t[0] = mp * V2;
-        t[1] = -0.5 * mp * A[E_PHI];
-        t[2] = 0.375 * mp * V2 * V2 + 1.5 * m0 * e * mp * V2 + 0.5 * m0 * m0 * e * e * mp
-               + 0.5 * m0 * e * mp * (3.0 * V2 - 7.0 * v0v - dv0 * dv * e * e);
-        t[3] = mp * (0.25 * (6.0 * V2 * A[E_PHI] - 7.0 * vPhv - A[E_GAM]) + m0 * e * A[E_PHI]
-                     + 0.5 * m0 * e * (G1 - mp * e));
+            for (int k = 0; k < NACCE; ++k) A[k] += part[(size_t)(s * NACCE + k) * P.cap + p];
+        efin_star(st, m, P.cap, p, 0, P.m0, P.eps2, A, gs[G_G1], t);
     }
     block_sum<NEB>(t, eblk + (size_t)blockIdx.x * NEB);
 }

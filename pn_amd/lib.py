@@ -68,6 +68,21 @@ def load():
         "pn1_profile_enable": (_I, [P, _I]),
         "pn1_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
         "pn1_util_copy": (_I, [P, ctypes.c_longlong, _I]),
+        "pn1_ens_create": (_I, [ctypes.POINTER(P), _I, _I, _I]),
+        "pn1_ens_destroy": (None, [P]),
+        "pn1_ens_set_params": (_I, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I]),
+        "pn1_ens_upload": (_I, [P, _I, ctypes.POINTER(_I), _D, _D, _D]),
+        "pn1_ens_download": (_I, [P, _D, _D]),
+        "pn1_ens_force": (_I, [P, _D]),
+        "pn1_ens_energy": (_I, [P, _D]),
+        "pn1_ens_step": (_I, [P, _I]),
+        "pn1_ens_sync": (_I, [P]),
+        "pn1_ens_last_error": (ctypes.c_char_p, [P]),
+        "pn1_ens_set_tuning": (_I, [P, _I, _I, _I]),
+        "pn1_ens_get_tuning": (_I, [P, ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                    ctypes.POINTER(_I)]),
+        "pn1_ens_profile_enable": (_I, [P, _I]),
+        "pn1_ens_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
         "pn1_check_input": (_I, [_D, _D, _D, _I, ctypes.c_double, ctypes.c_char_p, _I]),
         "pn1_ic_uniform_sphere": (_I, [_I, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, _D, _D, _D]),
@@ -186,6 +201,115 @@ class Context:
         k, ms = _I(), ctypes.c_double()
         self._chk(self._L.pn1_profile_read(self._h, ctypes.byref(k), ctypes.byref(ms)),
                   "pn1_profile_read")
+        return k.value, ms.value
+
+
+# ---------------------------------------------------------------- ensemble
+class Ensemble:
+    """Many independent clusters advanced together (pn1_ens).
+
+    Clusters are given as a list of (x, v, m) states, each with its BH
+    first; arrays returned by `force` / `download` are lists in the same
+    order, and `energy` returns an array [nclusters, 5] of
+    (kin, pot, pot_pn, pot_pn2, total)."""
+
+    def __init__(self, capacity, max_clusters, device=0):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.pn1_ens_create(ctypes.byref(h), int(capacity), int(max_clusters), int(device))
+        if rc != PN1_OK:
+            raise Pn1Error(rc, f"pn1_ens_create(capacity={capacity}, "
+                               f"max_clusters={max_clusters}, device={device})")
+        self._L, self._h = L, h
+        self.capacity, self.max_clusters = int(capacity), int(max_clusters)
+        self.offsets = np.zeros(1, dtype=np.int32)
+
+    def _chk(self, rc, what):
+        if rc != PN1_OK:
+            raise Pn1Error(rc, f"{what}: {self._L.pn1_ens_last_error(self._h).decode()}")
+
+    def close(self):
+        if self._h:
+            self._L.pn1_ens_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n(self):
+        return int(self.offsets[-1])
+
+    def split(self, a):
+        o = self.offsets
+        return [a[o[c]:o[c + 1]] for c in range(len(o) - 1)]
+
+    def set_params(self, eps, c=1.0, dt=2.0**-10, mode=FP64):
+        self._chk(self._L.pn1_ens_set_params(self._h, eps, c, dt, int(mode)),
+                  "pn1_ens_set_params")
+
+    def set_tuning(self, chunk=0, tile=0, variant=-1):
+        self._chk(self._L.pn1_ens_set_tuning(self._h, int(chunk), int(tile), int(variant)),
+                  "pn1_ens_set_tuning")
+
+    def get_tuning(self):
+        s, t, v = _I(), _I(), _I()
+        self._chk(self._L.pn1_ens_get_tuning(self._h, ctypes.byref(s), ctypes.byref(t),
+                                             ctypes.byref(v)), "pn1_ens_get_tuning")
+        return s.value, t.value, v.value
+
+    def upload(self, clusters):
+        sizes = [len(m) for _, _, m in clusters]
+        off = np.zeros(len(sizes) + 1, dtype=np.int32)
+        off[1:] = np.cumsum(sizes)
+        def cat(k, shape):
+            parts = [np.asarray(c[k], dtype=np.float64).reshape(shape) for c in clusters]
+            return _f64(np.concatenate(parts) if parts else np.zeros((1,) + shape[1:]))
+        x, v, m = cat(0, (-1, 3)), cat(1, (-1, 3)), cat(2, (-1,))
+        self._chk(self._L.pn1_ens_upload(self._h, len(sizes),
+                                         off.ctypes.data_as(ctypes.POINTER(_I)),
+                                         ptr(x), ptr(v), ptr(m)), "pn1_ens_upload")
+        self.offsets = off
+
+    def download(self):
+        x = np.empty((self.n, 3))
+        v = np.empty((self.n, 3))
+        self._chk(self._L.pn1_ens_download(self._h, ptr(x), ptr(v)), "pn1_ens_download")
+        return self.split(x), self.split(v)
+
+    def force(self):
+        a = np.empty((self.n, 3))
+        self._chk(self._L.pn1_ens_force(self._h, ptr(a)), "pn1_ens_force")
+        return self.split(a)
+
+    def energy(self):
+        e = np.empty((len(self.offsets) - 1, 5))
+        self._chk(self._L.pn1_ens_energy(self._h, ptr(e)), "pn1_ens_energy")
+        return e
+
+    def step(self, steps):
+        self._chk(self._L.pn1_ens_step(self._h, int(steps)), "pn1_ens_step")
+
+    def sync(self):
+        self._chk(self._L.pn1_ens_sync(self._h), "pn1_ens_sync")
+
+    def profile_enable(self, on=True):
+        self._chk(self._L.pn1_ens_profile_enable(self._h, 1 if on else 0),
+                  "pn1_ens_profile_enable")
+
+    def profile_read(self):
+        k, ms = _I(), ctypes.c_double()
+        self._chk(self._L.pn1_ens_profile_read(self._h, ctypes.byref(k), ctypes.byref(ms)),
+                  "pn1_ens_profile_read")
         return k.value, ms.value
 
 

@@ -1,0 +1,210 @@
+"""Ensemble API (pn1_ens_*): many clusters in one persistent pair launch.
+
+Every cluster of an ensemble must come out as if it had been run alone:
+  - with one j chunk per star (chunk >= the largest cluster) and a context
+    whose j range is not split (jsplit 1), the ensemble and the context
+    evaluate the same pair sums in the same order, so forces, energies and
+    whole steps agree bit for bit, in both precision modes;
+  - with the default chunk (512 records), FP64 forces agree with the
+    oracle to the same 1e-12 per-particle bound as the context API;
+  - the persistent kernel hands items to workgroups in whatever order they
+    ask, but every item writes its own slot: repeated evaluations are
+    bitwise identical.
+Cluster sizes cover the empty cluster (BH only), one star, sizes either
+side of the 64-lane wave and 256-star block, and both sides of a chunk.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from conftest import energy_atol, rel_err
+from pn_amd import lib as pn
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [0, 1, 2, 63, 64, 65, 255, 256, 257, 511, 512, 513, 1000, 1537]
+EPS, C, DT = 1e-3, 20.0, 2.0**-10
+
+
+def make_clusters(sizes=SIZES, seed=11):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, ns in enumerate(sizes):
+        m0 = float(rng.uniform(0.005, 0.05))
+        if ns < 4:
+            # the samplers centre the stars on the BH: with one star that
+            # puts it on the BH, which pn1 rejects
+            x = np.vstack([np.zeros(3), rng.uniform(-1, 1, (ns, 3))])
+            v = np.vstack([np.zeros(3), rng.normal(0, 0.3, (ns, 3))])
+            m = np.concatenate([[m0], np.full(ns, 1.0 / max(ns, 1))])
+            out.append((x, v, m))
+        elif k % 2:
+            out.append(pn.ic_plummer(ns, seed + k, a=1.0, m_bh=m0))
+        else:
+            out.append(pn.ic_uniform_sphere(ns, seed + k, R=1.0, Q=0.5, m_bh=m0, eps=EPS))
+    return out
+
+
+@pytest.fixture(scope="module")
+def clusters():
+    return make_clusters()
+
+
+@pytest.fixture(scope="module")
+def ens(clusters):
+    n = sum(len(m) for _, _, m in clusters)
+    e = pn.Ensemble(capacity=n + 64, max_clusters=len(clusters) + 4)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pn.Context(capacity=2048)
+    yield c
+    c.close()
+
+
+def single(ctx, cl, mode, what, steps=0):
+    ctx.set_tuning(1, 0, -1)
+    ctx.set_params(EPS, C, DT, mode)
+    ctx.upload(*cl)
+    if what == "force":
+        return ctx.force()
+    if what == "energy":
+        return ctx.energy()
+    ctx.step(steps)
+    return ctx.download()
+
+
+@pytest.mark.parametrize("mode", [pn.FP64, pn.MIXED])
+def test_force_bitwise_equals_single_cluster(ens, ctx, clusters, mode):
+    ens.set_tuning(4096, 0, -1)
+    ens.set_params(EPS, C, DT, mode)
+    ens.upload(clusters)
+    assert ens.get_tuning()[0] == 4096
+    got = ens.force()
+    for k, cl in enumerate(clusters):
+        want = single(ctx, cl, mode, "force")
+        diff = np.abs(got[k] - want).max()
+        print(f"mode {mode} cluster {k} n={len(cl[2])}: max |diff| {diff:.3e}")
+        assert np.array_equal(got[k], want), (mode, k, diff)
+
+
+def test_energy_bitwise_equals_single_cluster(ens, ctx, clusters):
+    ens.set_tuning(4096, 0, -1)
+    ens.set_params(EPS, C, DT, pn.FP64)
+    ens.upload(clusters)
+    got = ens.energy()
+    assert got.shape == (len(clusters), 5)
+    for k, cl in enumerate(clusters):
+        want = single(ctx, cl, pn.FP64, "energy")
+        assert np.array_equal(got[k], want), (k, got[k], want)
+
+
+@pytest.mark.parametrize("mode", [pn.FP64, pn.MIXED])
+def test_steps_bitwise_equal_single_cluster(ens, ctx, clusters, mode):
+    ens.set_tuning(4096, 0, -1)
+    ens.set_params(EPS, C, DT, mode)
+    ens.upload(clusters)
+    ens.step(3)
+    xs, vs = ens.download()
+    for k, cl in enumerate(clusters):
+        x, v = single(ctx, cl, mode, "step", steps=3)
+        assert np.array_equal(xs[k], x) and np.array_equal(vs[k], v), (mode, k)
+
+
+def test_default_chunk_fp64_vs_oracle(ens, clusters):
+    ens.set_tuning(0, 0, -1)
+    ens.set_params(EPS, C, DT, pn.FP64)
+    ens.upload(clusters)
+    assert ens.get_tuning()[0] == 512
+    got = ens.force()
+    for k, (x, v, m) in enumerate(clusters):
+        want = ob.orc_force(x, v, m, EPS, C)
+        err = rel_err(got[k], want)
+        print(f"cluster {k} n={len(m)}: max rel {err.max():.3e}")
+        assert err.max() <= 1e-12, (k, err.max())
+
+
+def test_default_chunk_energy_and_steps_vs_oracle(ens, clusters):
+    ens.set_tuning(0, 0, -1)
+    ens.set_params(EPS, C, DT, pn.FP64)
+    ens.upload(clusters)
+    e = ens.energy()
+    for k, (x, v, m) in enumerate(clusters):
+        want = ob.orc_energy_terms(x, v, m, EPS, C)
+        tol = energy_atol({"m": m, "eps": EPS}, want)
+        assert np.abs(e[k] - want).max() <= tol, (k, e[k], want)
+    ens.step(2)
+    xs, vs = ens.download()
+    for k, (x, v, m) in enumerate(clusters):
+        if len(m) > 600:
+            continue     # oracle time: the smaller clusters cover the path
+        xo, vo = ob.orc_q1_steps(x, v, m, EPS, C, DT, 2)
+        assert np.abs(xs[k] - xo).max() <= 1e-13 * max(1.0, np.abs(xo).max()), k
+        assert np.abs(vs[k] - vo).max() <= 1e-13 * max(1.0, np.abs(vo).max()), k
+
+
+@pytest.mark.parametrize("mode", [pn.FP64, pn.MIXED])
+def test_repeat_evaluations_bitwise_identical(ens, clusters, mode):
+    ens.set_tuning(128, 0, -1)
+    ens.set_params(EPS, C, DT, mode)
+    ens.upload(clusters)
+    a1 = np.concatenate(ens.force())
+    for _ in range(3):
+        a2 = np.concatenate(ens.force())
+        assert np.array_equal(a1, a2)
+
+
+def test_mixed_accuracy(ens, clusters):
+    ens.set_tuning(0, 0, -1)
+    ens.set_params(EPS, C, DT, pn.FP64)
+    ens.upload(clusters)
+    a64 = ens.force()
+    ens.set_params(EPS, C, DT, pn.MIXED)
+    a32 = ens.force()
+    for k in range(len(clusters)):
+        if len(clusters[k][2]) < 2:
+            continue
+        err = rel_err(a32[k], a64[k])
+        assert err[1:].max() <= 1e-5 and err[0] <= 1e-6, (k, err[1:].max(), err[0])
+
+
+def test_graph_replay_equals_eager_profiled(ens, clusters):
+    """pn1_ens_step replays a captured graph; with profiling on it launches
+    eagerly.  Both must give the same bits."""
+    ens.set_tuning(0, 0, -1)
+    ens.set_params(EPS, C, DT, pn.MIXED)
+    ens.upload(clusters)
+    ens.step(4)
+    xg, vg = ens.download()
+    ens.upload(clusters)
+    ens.profile_enable(True)
+    ens.step(4)
+    k, ms = ens.profile_read()
+    ens.profile_enable(False)
+    xe, ve = ens.download()
+    assert k == 12 and ms > 0.0
+    for a, b in zip(xg + vg, xe + ve):
+        assert np.array_equal(a, b)
+
+
+def test_upload_rejects_bad_layouts(ens, clusters):
+    x, v, m = clusters[3]
+    with pytest.raises(pn.Pn1Error):
+        ens.upload([])
+    with pytest.raises(pn.Pn1Error):
+        ens.upload([(x, v, m)] * (ens.max_clusters + 1))
+    bad = m.copy()
+    bad[0] = 0.0
+    with pytest.raises(pn.Pn1Error, match="cluster 1"):
+        ens.upload([(x, v, m), (x, v, bad)])
+    xl, vl, ml = clusters[-1]
+    big = ens.capacity // len(ml) + 1
+    assert big <= ens.max_clusters
+    with pytest.raises(pn.Pn1Error, match="capacity"):
+        ens.upload([(xl, vl, ml)] * big)
+    # the previous good state is still usable after a rejected upload
+    ens.upload(clusters[:2])
+    assert len(ens.force()) == 2
