@@ -7,7 +7,7 @@ of the reference's stage driver (Limit-sticky9.c:841-893): three force
 evaluations of n^2 interactions each (n = 131073 counts the black hole), plus
 the O(n) closed forms and the stage update, with the state resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--ens-steps K]
 
 Prints ONE JSON line.  `value` is the mixed-precision form's interactions/s
 (star-star pairs in FP32 on double-single positions, everything else FP64);
@@ -17,6 +17,12 @@ the same state.  The pair kernel's average launch duration comes from HIP
 events recorded around every pair launch on the library's stream during the
 timed steps.  Algorithmic work: 90 flops per interaction (FMA = 2,
 rsqrt = 1; Newtonian 20 + three-body 1PN 70).
+
+The line also carries an "ensemble" object: the batched workload of 256
+seeded Plummer clusters (star counts log-uniform in [256, 8192]) advanced
+together through the ensemble API for --ens-steps steps (default 128), its
+interactions/s (sum of n^2 per force evaluation), pair-kernel roofline and
+mixed-vs-FP64 error, against the same clusters stepped one at a time.
 """
 import argparse
 import json
@@ -32,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from pn_amd import lib as pn  # noqa: E402
+from pn_amd import workloads  # noqa: E402
 
 METRIC = ("interactions/s & % vector peak at N=128k, mixed vs FP64 GPU @ acc rel err"
           "≤1e-6")
@@ -40,6 +47,7 @@ FLOPS_PER_INTERACTION = 90
 PEAK_FP32 = 157.3e12   # MI355X vector FP32
 PEAK_FP64 = 78.6e12    # MI355X vector FP64
 CPU_SAMPLE_STARS = 16384
+ENS_TRAFFIC = "ensemble_pair_traffic_c5.json"   # measured at chunk 512
 
 
 def rel_err(a, ref):
@@ -80,9 +88,9 @@ def roofline(n, avg_ms, peak, traffic=None):
             "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4), "traffic": traffic}
 
 
-def load_traffic(kind):
-    """HBM bytes per pair launch from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "pair_kernel_traffic_n131073.json")
+def load_traffic(kind, name="pair_kernel_traffic_n131073.json"):
+    """HBM bytes per pair launch from a committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             return json.load(f).get(kind)
@@ -122,6 +130,95 @@ def cpu_baseline(x, v, m):
                       f"(n={n}, {n * n:.3e} interactions, {dt:.1f} s, gcc -O2, 1 thread, {cpu})"}
 
 
+def time_steps(obj, steps, warmup, eager=False):
+    """Wall time of `steps` steps after `warmup` (graph-replayed steps, or
+    with eager=True the same launches one by one)."""
+    obj.profile_enable(eager)
+    obj.step(warmup)
+    obj.sync()
+    t0 = time.perf_counter()
+    obj.step(steps)
+    obj.sync()
+    wall = time.perf_counter() - t0
+    obj.profile_enable(False)
+    return wall
+
+
+def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False):
+    """The batched workload: 256 seeded Plummer clusters (star counts
+    log-uniform in [256, 8192], BH masses uniform in [0.005, 0.05], c = 100,
+    eps = 1e-3, dt = 2^-10), all advanced by one pn1_ens_step per step (one
+    persistent pair launch per force evaluation, graph-captured steps),
+    against the same clusters advanced one after another through a single
+    context (pn1_step, also graph-captured)."""
+    W = workloads.ENSEMBLE
+    cl = workloads.ensemble()
+    inter = workloads.interactions(cl)
+    ntot = sum(len(m) for _, _, m in cl)
+    res = {}
+    with pn.Ensemble(capacity=ntot, max_clusters=len(cl)) as ens:
+        ens.set_tuning(chunk, tile, variant)
+        for mode, name in ((pn.FP64, "fp64"), (pn.MIXED, "mixed")):
+            ens.set_params(W["eps"], W["c"], W["dt"], mode)
+            ens.upload(cl)
+            e0 = ens.energy()
+            wall = time_steps(ens, steps, warmup, eager)
+            e1 = ens.energy()
+            ens.upload(cl)
+            ens.profile_enable(True)
+            ens.step(1)
+            k, ms = ens.profile_read()
+            ens.profile_enable(False)
+            drift = np.abs((e1[:, 4] - e0[:, 4]) / np.abs(e0[:, 4]))
+            res[name] = dict(interactions_per_s=3.0 * inter * steps / wall,
+                             ms_per_step=1e3 * wall / steps, pair_avg_ms=ms / max(k, 1),
+                             energy_drift_max=float(drift.max()),
+                             energy_drift_median=float(np.median(drift)))
+        ens.set_params(W["eps"], W["c"], W["dt"], pn.FP64)
+        ens.upload(cl)
+        a64 = np.concatenate(ens.force())
+        ens.set_params(W["eps"], W["c"], W["dt"], pn.MIXED)
+        a32 = np.concatenate(ens.force())
+        chunk_used, tile_used, var_used = ens.get_tuning()
+    err = rel_err(a32, a64)
+    sep_wall = 0.0
+    with pn.Context(capacity=max(len(m) for _, _, m in cl)) as ctx:
+        ctx.set_params(W["eps"], W["c"], W["dt"], pn.MIXED)
+        for x, v, m in cl:
+            ctx.upload(x, v, m)
+            sep_wall += time_steps(ctx, steps, warmup, eager)
+    sep = 3.0 * inter * steps / sep_wall
+    mixed = res["mixed"]
+    return {
+        "workload": f"{len(cl)} Plummer clusters, N log-uniform in [{W['n_min']}, {W['n_max']}] "
+                    f"(seed {W['seed']}), BH mass U[{W['m_bh_min']}, {W['m_bh_max']}], "
+                    f"c={W['c']:g}, eps={W['eps']:g}, dt=2^-10, {steps} steps, mixed",
+        "particles": ntot, "interactions_per_force": inter,
+        "interactions_per_s": mixed["interactions_per_s"],
+        "ms_per_step": mixed["ms_per_step"],
+        "separate_interactions_per_s": sep,
+        "ensemble_over_separate": mixed["interactions_per_s"] / sep,
+        "roofline": roofline_flops(inter, mixed["pair_avg_ms"], PEAK_FP32,
+                                   load_traffic("mixed", ENS_TRAFFIC) if chunk_used == 512 else None),
+        "fp64": {"interactions_per_s": res["fp64"]["interactions_per_s"],
+                 "roofline": roofline_flops(inter, res["fp64"]["pair_avg_ms"], PEAK_FP64,
+                                            load_traffic("fp64", ENS_TRAFFIC)
+                                            if chunk_used == 512 else None),
+                 "energy_drift_max": res["fp64"]["energy_drift_max"]},
+        "mixed_accuracy_vs_fp64": {"rel_err_max": float(err.max()),
+                                   "rel_err_p999": float(np.quantile(err, 0.999)),
+                                   "energy_drift_max": mixed["energy_drift_max"],
+                                   "energy_drift_median": mixed["energy_drift_median"]},
+        "tuning": {"chunk": chunk_used, "tile": tile_used, "variant": var_used},
+    }
+
+
+def roofline_flops(inter, avg_ms, peak, traffic=None):
+    achieved = FLOPS_PER_INTERACTION * inter / (avg_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak / 1e12,
+            "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4), "traffic": traffic}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +228,12 @@ def main():
     ap.add_argument("--jsplit", type=int, default=0)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--ens-steps", type=int, default=128,
+                    help="steps of the batched-ensemble workload (0 skips it)")
+    ap.add_argument("--ens-chunk", type=int, default=0)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel of the ensemble leg directly instead of "
+                         "replaying captured step graphs (for profilers)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB, 3 times (for "
                          "calibrating profiler byte counters), and exit")
@@ -192,6 +295,9 @@ def main():
                                "launches": mixed["pair_launches"]},
         "flops_per_interaction": FLOPS_PER_INTERACTION,
     }
+    if args.ens_steps > 0:
+        out["ensemble"] = run_ensemble(args.ens_steps, args.warmup, chunk=args.ens_chunk,
+                                       tile=args.tile, variant=args.variant, eager=args.eager)
     if not args.no_cpu:
         old = threading.stack_size(256 << 20)
         res = {}

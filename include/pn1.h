@@ -133,8 +133,10 @@ int  pn1_util_copy(pn1_ctx *ctx, long long bytes, int reps);
  * its first particle is its BH.  All clusters share eps, c, dt and mode.
  * One force evaluation of the ensemble is one persistent pair-kernel
  * launch: workgroups take (cluster, 256-star block, j chunk) items from
- * a device counter, largest first; each item writes its own partial sums,
- * so results do not depend on the order items run in. */
+ * eight per-XCD queues (whole clusters per queue, balanced by cost;
+ * largest items first; idle workgroups take from the other queues); each
+ * item writes its own partial sums, so results do not depend on the order
+ * items run in. */
 typedef struct pn1_ens pn1_ens;
 
 /* Up to `capacity` particles in total in up to `max_clusters` clusters. */
@@ -157,9 +159,10 @@ int  pn1_ens_energy(pn1_ens *ens, double *out);
 int  pn1_ens_step(pn1_ens *ens, int steps);
 int  pn1_ens_sync(pn1_ens *ens);
 const char *pn1_ens_last_error(const pn1_ens *ens);
-/* Work-item shape: j chunk length `chunk` (records per item, 0 = 512;
- * applies from the next upload), mixed compensation interval `tile` and
- * kernel `variant` as pn1_set_tuning. */
+/* Work-item shape: j chunk length `chunk` (records per item; 0 = chosen
+ * at upload so that each persistent workgroup takes about 8 items; applies
+ * from the next upload), mixed compensation interval `tile` and kernel
+ * `variant` as pn1_set_tuning. */
 int  pn1_ens_set_tuning(pn1_ens *ens, int chunk, int tile, int variant);
 int  pn1_ens_get_tuning(const pn1_ens *ens, int *chunk, int *tile,
                         int *variant);

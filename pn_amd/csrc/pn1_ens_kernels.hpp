@@ -7,24 +7,34 @@
 //   k_ens_prep   per-star j-records and per-block separable sums; blocks
 //                come from a block table, each inside one cluster
 //   k_ens_gsum   one workgroup per cluster: that cluster's separable sums;
-//                also resets the work counter of the pair kernel
+//                also resets the work counters of the pair kernel
 //   k_ens_pair*  ONE persistent launch for all clusters: workgroups take
-//                (cluster, i-block, j-chunk) items from an atomic counter,
-//                items sorted by decreasing cost on the host; every item
-//                writes its partial sums to its own slot, so the result
-//                does not depend on which workgroup ran which item
+//                (cluster, 256-star i-block, j-chunk) items from per-XCD
+//                queues (atomic counters), items sorted by decreasing cost
+//                on the host; every item writes its partial sums to its own
+//                slot, so the result does not depend on which workgroup ran
+//                which item
 //   k_ens_fin    per star, as k_fin, with its cluster's sums and BH
 //   k_ens_bh     one workgroup per cluster: its BH
 //
-// The pair arithmetic is the single-cluster kernels' (body64 / body32).
+// The pair arithmetic is the single-cluster kernels' (body64 / body32),
+// and so is the order of each lane's j loop: with one chunk per cluster
+// the results equal a single-cluster run with an unsplit j range bit for
+// bit.
 #pragma once
 #include "pn1_kernels.hpp"
 
 namespace pn1dev {
 
-// A pair work item: cluster, i-block (256 stars), j chunk.
-struct __align__(16) EItem {
-    int c, ib, s, pad;
+// A pair work item: one 256-star i-block of a cluster against one j chunk,
+// self-contained so that a workgroup starts it with one scalar load.
+//   off    the cluster's BH index; its stars are off + 1 .. off + nst
+//   ib     i-block (local stars ib * 256 ..)
+//   j0, j1 the chunk's particle range
+//   pslot  first partial slot of the chunk: poff[c] + s * nst
+struct __align__(32) EItem {
+    int off, nst, ib, j0, j1, pad;
+    long long pslot;
 };
 
 // Star block of the O(n) kernels: cluster and first local star index.
@@ -42,16 +52,58 @@ struct EConst {
 
 __device__ __forceinline__ int ens_nchunks(int nst, int L) { return (nst + L - 1) / L; }
 
+// Work queues, one per XCD.  The dispatcher deals workgroup ids to the 8
+// XCDs round-robin, so workgroup b runs on XCD b % 8.  Whole clusters are
+// assigned to queues (balanced by cost on the host), so the j-records and
+// lane state of a cluster are fetched into one XCD's L2.  Queue q holds
+// items [qoff[q], qoff[q + 1]) with counter[q] its next index; a
+// workgroup drains its own XCD's queue, then takes items from the others
+// in turn, and leaves when all 8 are empty.  The four waves of a
+// workgroup work on one item, so they read the same j-record stream
+// (through the scalar cache) in step.
+constexpr int NXCD = 8;
+
 // The next work item for this workgroup, identical in every lane (and
-// known to the compiler as uniform).  Returns -1 when the queue is empty.
-__device__ __forceinline__ int ens_take(unsigned int* counter, int nitems)
+// known to the compiler as uniform); -1 when every queue is empty.  `k`
+// (thread 0's count of exhausted queues) starts at 0.
+__device__ __forceinline__ int ens_take(unsigned int* counter, const int* __restrict__ qoff,
+                                        int& k)
 {
     __shared__ int s_item;
     __syncthreads();                         // every lane has read the last item
-    if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+    if (threadIdx.x == 0) {
+        int it = -1;
+        while (k < NXCD) {
+            const int q = ((int)blockIdx.x + k) & (NXCD - 1);
+            const int idx = qoff[q] + (int)atomicAdd(counter + q, 1u);
+            if (idx < qoff[q + 1]) {
+                it = idx;
+                break;
+            }
+            ++k;
+        }
+        s_item = it;
+    }
     __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(s_item);
-    return it < nitems ? it : -1;
+    return __builtin_amdgcn_readfirstlane(s_item);
+}
+
+// Lane geometry of an item: the lane's local star, its particle index
+// (clamped into the cluster for lanes past its end) and the wave's own
+// window [w0, w1) of the chunk (masked self pairs).
+struct EGeom {
+    int li, i, w0, w1;
+};
+
+__device__ __forceinline__ EGeom ens_geom(const EItem& w)
+{
+    EGeom g;
+    g.li = w.ib * WG + (int)threadIdx.x;
+    g.i = w.off + 1 + (g.li < w.nst ? g.li : w.nst - 1);
+    const int wb = w.off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
+    g.w0 = max(w.j0, wb);
+    g.w1 = min(w.j1, wb + 64);
+    return g;
 }
 
 template <int PP>
@@ -59,36 +111,30 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
                                                   const double* __restrict__ st,
                                                   double* __restrict__ part,
                                                   const EItem* __restrict__ items,
-                                                  const int* __restrict__ coff,
-                                                  const long long* __restrict__ poff,
-                                                  unsigned int* __restrict__ counter, Phys P,
-                                                  EConst E)
+                                                  unsigned int* __restrict__ counter,
+                                                  const int* __restrict__ qoff, Phys P, EConst E)
 {
+    int q = 0;
     for (;;) {
-        const int it = ens_take(counter, E.nitems);
+        const int it = ens_take(counter, qoff, q);
         if (it < 0) break;
         const EItem w = items[it];
-        const int off = coff[w.c], nst = coff[w.c + 1] - off - 1;
-        const int li = w.ib * WG + (int)threadIdx.x;          // local star index
-        const int i = off + 1 + (li < nst ? li : nst - 1);
-        const I64 I = load_i64(jr, st, P.cap, i, 0.0, P.eps2);
-        const int j0 = off + 1 + w.s * E.L, j1 = min(off + 1 + nst, j0 + E.L);
-        const int wb = off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
-        const int w0 = max(j0, wb), w1 = min(j1, wb + 64);
+        const EGeom g = ens_geom(w);
+        const I64 I = load_i64(jr, st, P.cap, g.i, 0.0, P.eps2);
         double a[NACC];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0;
-        if (w0 < w1) {
-            run64<false, PP>(jr, j0, w0, i, I, a, P.eps2);
-            run64<true, PP>(jr, w0, w1, i, I, a, P.eps2);
-            run64<false, PP>(jr, w1, j1, i, I, a, P.eps2);
+        if (g.w0 < g.w1) {
+            run64<false, PP>(jr, w.j0, g.w0, g.i, I, a, P.eps2);
+            run64<true, PP>(jr, g.w0, g.w1, g.i, I, a, P.eps2);
+            run64<false, PP>(jr, g.w1, w.j1, g.i, I, a, P.eps2);
         } else {
-            run64<false, PP>(jr, j0, j1, i, I, a, P.eps2);
+            run64<false, PP>(jr, w.j0, w.j1, g.i, I, a, P.eps2);
         }
-        if (li < nst) {
-            double* pp = part + poff[w.c] * NACC + li;
+        if (g.li < w.nst) {
+            double* pp = part + w.pslot * NACC + g.li;
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) pp[(size_t)(w.s * NACC + k) * nst] = a[k];
+            for (int k = 0; k < NACC; ++k) pp[(size_t)k * w.nst] = a[k];
         }
     }
 }
@@ -98,42 +144,37 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
                                                   const double* __restrict__ st,
                                                   float* __restrict__ part,
                                                   const EItem* __restrict__ items,
-                                                  const int* __restrict__ coff,
-                                                  const long long* __restrict__ poff,
-                                                  unsigned int* __restrict__ counter, Phys P,
-                                                  EConst E, int tile)
+                                                  unsigned int* __restrict__ counter,
+                                                  const int* __restrict__ qoff, Phys P, EConst E,
+                                                  int tile)
 {
+    int q = 0;
     for (;;) {
-        const int it = ens_take(counter, E.nitems);
+        const int it = ens_take(counter, qoff, q);
         if (it < 0) break;
         const EItem w = items[it];
-        const int off = coff[w.c], nst = coff[w.c + 1] - off - 1;
-        const int li = w.ib * WG + (int)threadIdx.x;
-        const int i = off + 1 + (li < nst ? li : nst - 1);
-        const I32 I = load_i32(jr, st, P.cap, i, 0.0, P.eps2);
-        const int j0 = off + 1 + w.s * E.L, j1 = min(off + 1 + nst, j0 + E.L);
-        const int wb = off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
-        const int w0 = max(j0, wb), w1 = min(j1, wb + 64);
+        const EGeom g = ens_geom(w);
+        const I32 I = load_i32(jr, st, P.cap, g.i, 0.0, P.eps2);
         float a[NACC], sN[3], cN[3];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
 #pragma unroll
         for (int k = 0; k < 3; ++k) sN[k] = cN[k] = 0.0f;
         int cnt = 0;
-        if (w0 < w1) {
-            run32<false, PP>(jr, j0, w0, i, I, a, P.eps2f, sN, cN, tile, cnt);
-            run32<true, PP>(jr, w0, w1, i, I, a, P.eps2f, sN, cN, tile, cnt);
-            run32<false, PP>(jr, w1, j1, i, I, a, P.eps2f, sN, cN, tile, cnt);
+        if (g.w0 < g.w1) {
+            run32<false, PP>(jr, w.j0, g.w0, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<true, PP>(jr, g.w0, g.w1, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<false, PP>(jr, g.w1, w.j1, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
         } else {
-            run32<false, PP>(jr, j0, j1, i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<false, PP>(jr, w.j0, w.j1, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
         }
         fold_newton(a, sN, cN);
-        if (li < nst) {
-            float* pp = part + poff[w.c] * (NACC + 3) + (size_t)w.s * (NACC + 3) * nst + li;
+        if (g.li < w.nst) {
+            float* pp = part + w.pslot * (NACC + 3) + g.li;
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) pp[(size_t)k * nst] = k <= A_NZ ? sN[k] : a[k];
+            for (int k = 0; k < NACC; ++k) pp[(size_t)k * w.nst] = k <= A_NZ ? sN[k] : a[k];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pp[(size_t)(NACC + k) * nst] = cN[k];
+            for (int k = 0; k < 3; ++k) pp[(size_t)(NACC + k) * w.nst] = cN[k];
         }
     }
 }
@@ -176,7 +217,7 @@ __global__ __launch_bounds__(WG) void k_ens_gsum(const double* __restrict__ gblk
                                                 unsigned int* __restrict__ counter)
 {
     const int c = blockIdx.x;
-    if (c == 0 && threadIdx.x == 0) counter[0] = 0u;
+    if (c == 0 && threadIdx.x < NXCD) counter[threadIdx.x] = 0u;
     double g[NGS];
     cluster_rows_sum<NGS>(gblk, cblk, c, g);
     block_sum<NGS>(g, gs + (size_t)c * NGS);
@@ -237,40 +278,35 @@ __global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ j
                                                    const double* __restrict__ st,
                                                    double* __restrict__ part,
                                                    const EItem* __restrict__ items,
-                                                   const int* __restrict__ coff,
-                                                   const long long* __restrict__ poff,
-                                                   unsigned int* __restrict__ counter, Phys P,
+                                                   unsigned int* __restrict__ counter,
+                                                   const int* __restrict__ qoff, Phys P,
                                                    EConst E)
 {
+    int q = 0;
     for (;;) {
-        const int it = ens_take(counter, E.nitems);
+        const int it = ens_take(counter, qoff, q);
         if (it < 0) break;
         const EItem w = items[it];
-        const int off = coff[w.c], nst = coff[w.c + 1] - off - 1;
-        const int li = w.ib * WG + (int)threadIdx.x;
-        const int i = off + 1 + (li < nst ? li : nst - 1);
+        const EGeom g = ens_geom(w);
         IE64 I;
-        I.x = jr[i].x; I.y = jr[i].y; I.z = jr[i].z;
-        I.ux = st[3 * P.cap + i] / PN1_SQ15;
-        I.uy = st[4 * P.cap + i] / PN1_SQ15;
-        I.uz = st[5 * P.cap + i] / PN1_SQ15;
-        const int j0 = off + 1 + w.s * E.L, j1 = min(off + 1 + nst, j0 + E.L);
-        const int wb = off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
-        const int w0 = max(j0, wb), w1 = min(j1, wb + 64);
+        I.x = jr[g.i].x; I.y = jr[g.i].y; I.z = jr[g.i].z;
+        I.ux = st[3 * P.cap + g.i] / PN1_SQ15;
+        I.uy = st[4 * P.cap + g.i] / PN1_SQ15;
+        I.uz = st[5 * P.cap + g.i] / PN1_SQ15;
         double a[NACCE];
 #pragma unroll
         for (int k = 0; k < NACCE; ++k) a[k] = 0.0;
-        if (w0 < w1) {
-            erun64<false>(jr, j0, w0, i, I, a, P.eps2);
-            erun64<true>(jr, w0, w1, i, I, a, P.eps2);
-            erun64<false>(jr, w1, j1, i, I, a, P.eps2);
+        if (g.w0 < g.w1) {
+            erun64<false>(jr, w.j0, g.w0, g.i, I, a, P.eps2);
+            erun64<true>(jr, g.w0, g.w1, g.i, I, a, P.eps2);
+            erun64<false>(jr, g.w1, w.j1, g.i, I, a, P.eps2);
         } else {
-            erun64<false>(jr, j0, j1, i, I, a, P.eps2);
+            erun64<false>(jr, w.j0, w.j1, g.i, I, a, P.eps2);
         }
-        if (li < nst) {
-            double* pp = part + poff[w.c] * NACCE + li;
+        if (g.li < w.nst) {
+            double* pp = part + w.pslot * NACCE + g.li;
 #pragma unroll
-            for (int k = 0; k < NACCE; ++k) pp[(size_t)(w.s * NACCE + k) * nst] = a[k];
+            for (int k = 0; k < NACCE; ++k) pp[(size_t)k * w.nst] = a[k];
         }
     }
 }

@@ -1,6 +1,6 @@
 """Launch-shape tuner for the pair kernels (runs on the GPU through the C ABI).
 
-    python -m pn_amd.tune [--workload plummer|sphere] [--nstars N] [--seed S]
+    python -m pn_amd.tune [--workload plummer|sphere|ensemble] [--nstars N] [--seed S]
                           [--m0 M] [--eps E] [--c C] [--mode fp64|mixed|both]
                           [--reps R] [--json OUT]
 
@@ -10,7 +10,9 @@ evaluations of the same state, and for the mixed form measures the
 per-particle acceleration error against the FP64 form.  It prints one row
 per shape and the fastest shape per mode whose mixed error stays within
 1e-6 (max over stars), and can write everything as JSON.  The library's
-automatic choice (pn1_set_tuning(ctx, 0, 0, -1)) is listed first.
+automatic choice (pn1_set_tuning(ctx, 0, 0, -1)) is listed first.  With
+--workload ensemble it sweeps the ensemble API's work-item chunk length
+on the batched workload (256 Plummer clusters) instead.
 """
 import argparse
 import itertools
@@ -75,9 +77,42 @@ def sweep(ctx, x, v, m, eps, c, modes, reps=3, jsplits=(8, 16, 32, 64, 128),
     return rows, best
 
 
+def ens_sweep(ens, clusters, eps, c, mode, reps=3, chunks=(64, 128, 256, 512, 1024, 2048),
+              tiles=(16, 32), variants=(0, 1), out=sys.stdout):
+    """Ensemble form: times the persistent pair kernel per work-item chunk
+    length (and, mixed, compensation interval and variant)."""
+    inter = sum(float(len(m)) ** 2 for _, _, m in clusters)
+    rows = []
+    name = "fp64" if mode == pn.FP64 else "mixed"
+    tl = (0,) if mode == pn.FP64 else tiles
+    for shape in [(0, 0, -1)] + list(itertools.product(chunks, tl, variants)):
+        ens.set_tuning(*shape)
+        ens.set_params(eps, c, 2.0**-10, mode)
+        ens.upload(clusters)
+        ens.profile_enable(True)
+        ens.force()
+        ens.profile_enable(True)
+        for _ in range(reps):
+            ens.force()
+        k, ms = ens.profile_read()
+        ens.profile_enable(False)
+        used = ens.get_tuning()
+        ms /= max(k, 1)
+        row = dict(mode=name, requested=list(shape), chunk=used[0], tile=used[1],
+                   variant=used[2], ms=ms, interactions_per_s=inter / (ms * 1e-3))
+        rows.append(row)
+        print(f"{name:5s} chunk={used[0]:5d} tile={used[1]:3d} variant={used[2]} "
+              f"{'(auto)' if shape == (0, 0, -1) else '      '} {ms:9.3f} ms "
+              f"{row['interactions_per_s']:.3e} int/s", file=out, flush=True)
+    b = min(rows, key=lambda r: r["ms"])
+    print(f"best {name}: chunk={b['chunk']} tile={b['tile']} variant={b['variant']} "
+          f"{b['ms']:.3f} ms", file=out, flush=True)
+    return rows, b
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--workload", choices=("plummer", "sphere"), default="plummer")
+    ap.add_argument("--workload", choices=("plummer", "sphere", "ensemble"), default="plummer")
     ap.add_argument("--nstars", type=int, default=131072)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--m0", type=float, default=0.05)
@@ -87,6 +122,24 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--json", default=None)
     args = ap.parse_args(argv)
+    if args.workload == "ensemble":
+        from . import workloads
+        W = workloads.ENSEMBLE
+        clusters = workloads.ensemble()
+        ntot = sum(len(m) for _, _, m in clusters)
+        modes = {"fp64": (pn.FP64,), "mixed": (pn.MIXED,), "both": (pn.FP64, pn.MIXED)}[args.mode]
+        print(f"# ensemble of {len(clusters)} Plummer clusters ({ntot} particles), "
+              f"eps={W['eps']} c={W['c']}, {args.reps} timed force evaluations per shape",
+              flush=True)
+        result = {}
+        with pn.Ensemble(ntot, len(clusters)) as ens:
+            for mode in modes:
+                rows, best = ens_sweep(ens, clusters, W["eps"], W["c"], mode, reps=args.reps)
+                result["fp64" if mode == pn.FP64 else "mixed"] = dict(rows=rows, best=best)
+        if args.json:
+            with open(args.json, "w") as f:
+                json.dump({"workload": vars(args), **result}, f, indent=1)
+        return
     if args.workload == "plummer":
         x, v, m = pn.ic_plummer(args.nstars, args.seed, a=1.0, m_bh=args.m0)
     else:

@@ -5,8 +5,9 @@ Every cluster of an ensemble must come out as if it had been run alone:
     whose j range is not split (jsplit 1), the ensemble and the context
     evaluate the same pair sums in the same order, so forces, energies and
     whole steps agree bit for bit, in both precision modes;
-  - with the default chunk (512 records), FP64 forces agree with the
-    oracle to the same 1e-12 per-particle bound as the context API;
+  - with the automatic chunk (256 to 512 records, several chunks
+    per star for the larger clusters), FP64 forces agree with the oracle
+    to the same 1e-12 per-particle bound as the context API;
   - the persistent kernel hands items to workgroups in whatever order they
     ask, but every item writes its own slot: repeated evaluations are
     bitwise identical.
@@ -118,7 +119,8 @@ def test_default_chunk_fp64_vs_oracle(ens, clusters):
     ens.set_tuning(0, 0, -1)
     ens.set_params(EPS, C, DT, pn.FP64)
     ens.upload(clusters)
-    assert ens.get_tuning()[0] == 512
+    L = ens.get_tuning()[0]
+    assert 256 <= L <= 512 and L % 64 == 0
     got = ens.force()
     for k, (x, v, m) in enumerate(clusters):
         want = ob.orc_force(x, v, m, EPS, C)
