@@ -123,6 +123,30 @@ int  pn1_profile_read(pn1_ctx *ctx, int *launches, double *total_ms);
  * HBM traffic for calibrating profiler byte counters. */
 int  pn1_util_copy(pn1_ctx *ctx, long long bytes, int reps);
 
+/* ---- adjoint (FP64) ---------------------------------------------------- */
+/* VJP of one force evaluation at the resident state: for a cotangent u
+ * (AoS [n][3], one 3-vector per particle's acceleration) returns
+ * gx = sum_p u_p . d a_p / d x (AoS [n][3]), gv likewise for v, and
+ * *gm0 = sum_p u_p . d a_p / d m[0] (gm0 may be NULL).  No counterpart in
+ * the reference (it has no derivatives); the derivative of force() as
+ * written, Limit-sticky9.c:21-424, at this context's c. */
+int  pn1_force_vjp(pn1_ctx *ctx, const double *u, double *gx, double *gv,
+                   double *gm0);
+/* Gradient of a loss of the state after `steps` steps of the stage driver
+ * (Limit-sticky9.c:841-893) with respect to the initial state and the BH
+ * mass, by the discrete adjoint of the driver.  pn1_adjoint_forward steps
+ * the resident state (FP64 kernels whatever the mode) and keeps a
+ * checkpoint every `interval` steps; the final state stays resident (read
+ * it with pn1_download, evaluate the loss and its gradient on the host).
+ * pn1_adjoint_backward takes that gradient (gx, gv: d loss / d final x, v,
+ * AoS [n][3]) and returns d loss / d initial x, v (gx0, gv0) and d loss /
+ * d m[0] through the dynamics (*gm0, may be NULL).  It recomputes each
+ * segment from its checkpoint and leaves the initial state resident.  It
+ * fails with PN1_EINVAL if the state or eps, c, dt changed in between. */
+int  pn1_adjoint_forward(pn1_ctx *ctx, int steps, int interval);
+int  pn1_adjoint_backward(pn1_ctx *ctx, const double *gx, const double *gv,
+                          double *gx0, double *gv0, double *gm0);
+
 /* ====================================================================== */
 /* 2b. Ensemble API: many independent clusters (each a BH + its stars,    */
 /*     as one run of Limit-sticky9.c) advanced together on one device.    */

@@ -32,4 +32,11 @@ void orc_energy(double t, const double *x, const double *v, const double *m,
 void orc_q1_steps(double *x, double *v, const double *m, int n,
                   double eps, double c, double dt, int steps);
 
+/* Complex-step versions (ls9_oracle_cs.c): the same code on double
+ * complex arrays (real and imaginary parts interleaved); eps, c, dt real. */
+void orc_force_cs(const double *x, const double *v, const double *m,
+                  double eps, double c, double *a, int n);
+void orc_q1_steps_cs(double *x, double *v, const double *m, int n,
+                     double eps, double c, double dt, int steps);
+
 #endif

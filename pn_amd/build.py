@@ -54,9 +54,15 @@ def build(verbose=False):
     host_o = os.path.join(BUILD, "pn1_host.o")
     api_o = os.path.join(BUILD, "pn1_api.o")
     _run(["gcc", *C_FLAGS, *inc, "-c", os.path.join(CSRC, "pn1_host.c"), "-o", host_o], verbose)
+    adj_o = os.path.join(BUILD, "pn1_adj.o")
     _run([hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "pn1_api.hip"), "-o", api_o], verbose)
+    # the adjoint's dual-number kernels: finite values, no signed zeros, so
+    # products with zero seed tangents fold away (see pn1_adj.hip)
+    _run([hipcc, *HIP_FLAGS, "-ffinite-math-only", "-fno-signed-zeros", *inc, "-c",
+          os.path.join(CSRC, "pn1_adj.hip"), "-o", adj_o], verbose)
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, host_o, "-o", tmp], verbose)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, adj_o, host_o, "-o", tmp],
+         verbose)
     os.replace(tmp, LIB)
     _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "valu_rates.hip"), "-o", VALU_RATES], verbose)
     sticky = os.path.join(CSRC, "sticky9.c")

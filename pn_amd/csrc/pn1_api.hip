@@ -23,40 +23,13 @@
 #include <vector>
 
 #include "pn1.h"
+#include "pn1_ctx.hpp"
 #include "pn1_ens_kernels.hpp"
 #include "pn1_kernels.hpp"
 
 using namespace pn1dev;
 
-struct pn1_ctx {
-    int dev = -1;
-    hipStream_t st = nullptr;
-    int cap = 0, n = 0;
-    bool have_state = false;
-    double eps = 1e-3, c = 1.0, dt = 1.0 / 1024.0;
-    int mode = PN1_FP64;
-    double m0 = 0.0;            // host copy of m[0]
-    int jsplit_req = 0, tile_req = 0, variant_req = -1;
-    // launch plan for the current n
-    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 16;
-    // device buffers
-    double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr;
-    double *m = nullptr, *acc = nullptr, *gblk = nullptr, *gs = nullptr;
-    double *bhblk = nullptr, *eblk = nullptr, *eout = nullptr;
-    JRec64* j64 = nullptr;
-    JRec32* j32 = nullptr;
-    void* part = nullptr;
-    size_t part_bytes = 0;
-    double* h_stage = nullptr;  // pinned, 7 * cap doubles
-    hipGraphExec_t gexec = nullptr;
-    bool prof = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    int prof_launches = 0;
-    double prof_ms = 0.0;
-    char err[512] = {0};
-};
-
-namespace {
+namespace pn1host {
 
 int seterr(pn1_ctx* c, int code, const char* fmt, ...)
 {
@@ -68,13 +41,6 @@ int seterr(pn1_ctx* c, int code, const char* fmt, ...)
     }
     return code;
 }
-
-#define HIPCHK(ctx, call)                                                                 \
-    do {                                                                                  \
-        hipError_t e_ = (call);                                                           \
-        if (e_ != hipSuccess)                                                             \
-            return seterr((ctx), PN1_EDEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
-    } while (0)
 
 template <typename T>
 int dalloc(pn1_ctx* c, T** p, size_t count)
@@ -271,7 +237,9 @@ int fetch3(pn1_ctx* c, const double* dsrc, double* out)
     return PN1_OK;
 }
 
-}  // namespace
+}  // namespace pn1host
+
+using namespace pn1host;
 
 // =========================================================================
 // context API
@@ -286,6 +254,7 @@ void pn1_destroy(pn1_ctx* c)
     if (c->dev >= 0) (void)hipSetDevice(c->dev);
     if (c->st) (void)hipStreamSynchronize(c->st);
     drop_graph(c);
+    adj_free(c);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -362,6 +331,7 @@ int pn1_set_params(pn1_ctx* c, double eps, double cl, double dt, int mode)
     if (!(dt > 0.0) || !std::isfinite(dt)) return seterr(c, PN1_EINVAL, "dt must be > 0");
     if (mode != PN1_FP64 && mode != PN1_MIXED) return seterr(c, PN1_EINVAL, "unknown mode %d", mode);
     if (eps != c->eps || cl != c->c || dt != c->dt || mode != c->mode) drop_graph(c);
+    if (eps != c->eps || cl != c->c || dt != c->dt) ++c->gen;
     c->eps = eps;
     c->c = cl;
     c->dt = dt;
@@ -419,6 +389,7 @@ int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, in
     HIPCHK(c, hipMemcpyAsync(c->m, h + 6 * cap, n * sizeof(double), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     if (n != c->n || m[0] != c->m0) drop_graph(c);
+    ++c->gen;
     c->n = n;
     c->m0 = m[0];
     c->have_state = true;

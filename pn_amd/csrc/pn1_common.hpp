@@ -1,0 +1,145 @@
+// pn1_common.hpp -- definitions shared by the device code of libpn1
+// (force, energy, ensemble and adjoint kernels): launch constants, the
+// j-record layouts, run constants and reduction / mapping helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pn1dev {
+
+constexpr int WG = 256;     // threads per workgroup (4 waves of 64)
+constexpr int NACC = 16;    // force pair accumulators
+constexpr int NACCE = 5;    // energy pair accumulators
+constexpr int NGS = 7;      // separable O(n) sums
+constexpr int NEB = 4;      // energy per-block partials
+
+// sqrt(3/2): stored velocities are pre-scaled by it, so that the pair term
+// 1.5 (v_j.r)^2 w^2 is a plain square of (s_j.r) w with s_j = sqrt(3/2) v_j.
+#define PN1_SQ15 1.2247448713915890491
+// 4 / sqrt(3/2) and 3 / sqrt(3/2) = sqrt(6)
+#define PN1_4OSQ15 3.2659863237109041
+#define PN1_SQ6 2.4494897427831781
+
+// force pair accumulators (per star i, sums over stars j != i; t = |r|^2 - D2_i,
+// s_j = sqrt(3/2) v_j, K_u = m_j w^3 (4 v_i.r - 3 v_j.r)):
+//   N    = sum m_j w^3 r                                      (Newton)
+//   Rr   = sum K_r r,  K_r = m_j w^3 [m0(4e_i + 1.25e_j + 0.25e_j^3 t)
+//                                      + |v_i|^2 - 2|u|^2 + 1.5 (v_j.r)^2 w^2]
+//          using |v_i|^2 - 2|u|^2 = -|v_i|^2 - 2|v_j|^2 + 4 v_i.v_j; the
+//          i-only part al_i = 4 m0 e_i - |v_i|^2 is left out of the pair sum
+//          and added by k_fin as al_i N
+//   U    = sum K_u s_j,  KS = sum K_u      (sum K_u u = U / sqrt(3/2) - v_i KS)
+//   T3   = sum 3.5 m0 m_j e_j^3 w d_j      (k_fin subtracts e_i times its w-free sum)
+//   phi  = sum m_j w,  xi = sum m_j w^3 (D2_i - D2_j),  zeta = sum m_j 0.25 m0 e_j^3 t
+// zeta shares t with K_r, as the reference's two e_j^3 terms share
+// r1b2 - rbc2 (Limit-sticky9.c:302, 305); the two cancel strongly when star
+// j is close to the BH, and the shared rounding cancels with them.  The
+// (w - e_i) c_j term (:307-308) is summed over d_j = x_j - x_BH, never over
+// r + d_i, which would cancel catastrophically in the same situation.
+enum { A_NX, A_NY, A_NZ, A_RX, A_RY, A_RZ, A_UX, A_UY, A_UZ, A_TX, A_TY, A_TZ,
+       A_KS, A_PHI, A_XI, A_ZE };
+// energy pair accumulators: phi, sum m_j w s_j, sum m_j w^3 (r.v_i)(r.s_j)
+enum { E_PHI, E_PVX, E_PVY, E_PVZ, E_GAM };
+// separable sums over stars: sum m e, sum m e^3 d, sum m v
+enum { G_G1, G_M1X, G_M1Y, G_M1Z, G_PX, G_PY, G_PZ };
+
+// j-record, FP64 form, 128 bytes (two 16-dword scalar loads):
+//   x            position                 s = sqrt(3/2) v
+//   m, D2        mass, |d|^2, d = x - x_BH
+//   A            1.25 m0 e - 2 |v|^2       B = 0.25 m0 e^3
+//   Q            3.5 m0 m e^3              MB = m B
+//   d            x - x_BH                  e = (|d|^2 + eps^2)^-1/2
+struct __align__(16) JRec64 {
+    double x, y, z, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e;
+};
+// j-record, mixed form: double-single position (hi, lo), the rest rounded
+// to FP32.  80 bytes.
+struct __align__(16) JRec32 {
+    float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e, pad;
+};
+static_assert(sizeof(JRec64) == 128, "JRec64 layout");
+static_assert(sizeof(JRec32) == 80, "JRec32 layout");
+
+// Everything a kernel needs about the run, passed by value.
+struct Phys {
+    double eps, eps2, lam, dt, m0;
+    double ws;       // 1/sqrt(eps^2): the reference's self-pair w
+    float eps2f, m0f;
+    int n, cap;      // particles incl. the BH; SoA stride
+};
+
+// ---------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------
+// FP64 reciprocal square root.  v_rsq_f64 alone is good to ~2^-24
+// relative (measured on gfx950), so one correction of third order follows:
+// with r = 1 - x y^2, 1/sqrt(x) = y (1 - r)^(-1/2) = y (1 + r/2 + 3r^2/8 + ...),
+// truncation ~ (5/16) r^3 ~ 1e-22; the result is within ~1 ulp.
+__device__ __forceinline__ double rsq64(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double h = x * y;
+    const double r = __builtin_fma(-h, y, 1.0);
+    const double t = __builtin_fma(0.375, r, 0.5);
+    return __builtin_fma(y * r, t, y);
+}
+
+// Fixed-order workgroup sum of N doubles; thread 0 writes out[0..N).
+// Wave level: xor butterfly (every lane ends with the same bits); then the
+// four wave totals are added in wave order.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* out)
+{
+    __shared__ double sh[N][WG / 64];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double t = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        v[k] = t;
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) sh[k][wid] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    }
+}
+
+// Workgroup -> (j chunk s, i block ib).  When S divides 8, consecutive
+// workgroup ids (which the dispatcher deals round-robin to the 8 XCDs) are
+// arranged so that every workgroup on XCD x works on chunk x % S.
+__device__ __forceinline__ void map_block(int bid, int S, int& s, int& ib)
+{
+    if (S <= 8 && (8 % S) == 0) {
+        const int xcd = bid & 7, k = bid >> 3, per = 8 / S;
+        s = xcd % S;
+        ib = k * per + xcd / S;
+    } else {
+        s = bid % S;
+        ib = bid / S;
+    }
+}
+
+// Chunk bounds of the j range [1, n) split in S parts, and the part of it
+// that can contain j == i for this wave's 64 lanes.
+struct JRange {
+    int j0, j1, w0, w1;
+};
+__device__ __forceinline__ JRange j_range(int s, int S, int n, int ib)
+{
+    JRange R;
+    const int nst = n - 1;
+    const int clen = (nst + S - 1) / S;
+    R.j0 = 1 + s * clen;
+    R.j1 = min(n, R.j0 + clen);
+    const int wb = ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63) + 1;
+    R.w0 = max(R.j0, wb);
+    R.w1 = min(R.j1, wb + 64);
+    return R;
+}
+
+}  // namespace pn1dev

@@ -68,6 +68,9 @@ def load():
         "pn1_profile_enable": (_I, [P, _I]),
         "pn1_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
         "pn1_util_copy": (_I, [P, ctypes.c_longlong, _I]),
+        "pn1_force_vjp": (_I, [P, _D, _D, _D, _D]),
+        "pn1_adjoint_forward": (_I, [P, _I, _I]),
+        "pn1_adjoint_backward": (_I, [P, _D, _D, _D, _D, _D]),
         "pn1_ens_create": (_I, [ctypes.POINTER(P), _I, _I, _I]),
         "pn1_ens_destroy": (None, [P]),
         "pn1_ens_set_params": (_I, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _I]),
@@ -202,6 +205,31 @@ class Context:
         self._chk(self._L.pn1_profile_read(self._h, ctypes.byref(k), ctypes.byref(ms)),
                   "pn1_profile_read")
         return k.value, ms.value
+
+    # ---- adjoint
+    def force_vjp(self, u):
+        """(gx, gv, gm0): gradient of sum_p u_p . a_p at the resident state."""
+        u = _f64(u, (self.n, 3))
+        gx = np.empty((self.n, 3))
+        gv = np.empty((self.n, 3))
+        gm0 = ctypes.c_double()
+        self._chk(self._L.pn1_force_vjp(self._h, ptr(u), ptr(gx), ptr(gv), ctypes.byref(gm0)),
+                  "pn1_force_vjp")
+        return gx, gv, gm0.value
+
+    def adjoint_forward(self, steps, interval=16):
+        self._chk(self._L.pn1_adjoint_forward(self._h, int(steps), int(interval)),
+                  "pn1_adjoint_forward")
+
+    def adjoint_backward(self, gx, gv):
+        """(gx0, gv0, gm0) for the final-state gradient (gx, gv)."""
+        gx, gv = _f64(gx, (self.n, 3)), _f64(gv, (self.n, 3))
+        gx0 = np.empty((self.n, 3))
+        gv0 = np.empty((self.n, 3))
+        gm0 = ctypes.c_double()
+        self._chk(self._L.pn1_adjoint_backward(self._h, ptr(gx), ptr(gv), ptr(gx0), ptr(gv0),
+                                               ctypes.byref(gm0)), "pn1_adjoint_backward")
+        return gx0, gv0, gm0.value
 
 
 # ---------------------------------------------------------------- ensemble

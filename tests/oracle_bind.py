@@ -38,6 +38,8 @@ def oracle():
         L.orc_initial_energy.argtypes = [_D, _D, _D, _int, _D, _dbl, _dbl]
         L.orc_energy.argtypes = [_dbl, _D, _D, _D, _int, _dbl, _dbl, _dbl]
         L.orc_q1_steps.argtypes = [_D, _D, _D, _int, _dbl, _dbl, _dbl, _int]
+        L.orc_force_cs.argtypes = [_D, _D, _D, _dbl, _dbl, _D, _int]
+        L.orc_q1_steps_cs.argtypes = [_D, _D, _D, _int, _dbl, _dbl, _dbl, _int]
         _orc = L
     return _orc
 
@@ -145,6 +147,68 @@ def orc_q1_steps(x, v, m, eps, c, dt, steps):
     x, v = x.copy(), v.copy()
     oracle().orc_q1_steps(ptr(x), ptr(v), ptr(m), m.shape[0], eps, c, dt, steps)
     return x, v
+
+
+# ---------------------------------------------------------------- complex step
+CS_H = 1e-30     # imaginary step: far below any rounding of the real parts
+
+
+def _cplx(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _cptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def orc_force_cs(x, v, m, eps, c=1.0):
+    """The oracle's force on complex inputs (complex arrays in and out)."""
+    x, v, m = _cplx(x), _cplx(v), _cplx(m)
+    a = np.zeros_like(x)
+    oracle().orc_force_cs(_cptr(x), _cptr(v), _cptr(m), eps, c, _cptr(a), m.shape[0])
+    return a
+
+
+def orc_q1_steps_cs(x, v, m, eps, c, dt, steps):
+    x, v, m = _cplx(x).copy(), _cplx(v).copy(), _cplx(m)
+    oracle().orc_q1_steps_cs(_cptr(x), _cptr(v), _cptr(m), m.shape[0], eps, c, dt, steps)
+    return x, v
+
+
+def _perturbed(x, v, m, coord):
+    """Complex copies of (x, v, m) with i*CS_H added to one input:
+    coord = ("x" | "v", particle, axis) or ("m0",)."""
+    xc, vc, mc = _cplx(x).copy(), _cplx(v).copy(), _cplx(m).copy()
+    if coord[0] == "x":
+        xc[coord[1], coord[2]] += 1j * CS_H
+    elif coord[0] == "v":
+        vc[coord[1], coord[2]] += 1j * CS_H
+    else:
+        mc[0] += 1j * CS_H
+    return xc, vc, mc
+
+
+def cs_force_vjp(x, v, m, u, eps, c, coords):
+    """d(sum_p u_p . a_p)/d(coord) for each coord, by complex step on the
+    oracle's force."""
+    out = []
+    for co in coords:
+        a = orc_force_cs(*_perturbed(x, v, m, co), eps, c)
+        out.append(float(np.sum(u * a.imag)) / CS_H)
+    return np.array(out)
+
+
+def cs_trajectory_grad(x, v, m, eps, c, dt, steps, loss_grad, coords):
+    """d loss(final state)/d(coord) for each coord: complex step through the
+    oracle's stage driver, contracted with loss_grad = (dL/dx_T, dL/dv_T)
+    evaluated at the unperturbed final state (exact for the first-order
+    term, which is all the imaginary part carries)."""
+    gx, gv = loss_grad
+    out = []
+    for co in coords:
+        xt, vt = orc_q1_steps_cs(*_perturbed(x, v, m, co), eps, c, dt, steps)
+        out.append(float(np.sum(gx * xt.imag) + np.sum(gv * vt.imag)) / CS_H)
+    return np.array(out)
 
 
 # ---------------------------------------------------------------- reference calls

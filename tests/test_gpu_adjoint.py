@@ -1,0 +1,217 @@
+"""Adjoint of the force and of the stage driver (pn1_force_vjp,
+pn1_adjoint_forward / _backward) against the complex-step oracle and
+finite differences.
+
+Complex step (oracle/ls9_oracle_cs.c) differentiates the oracle's own code
+exactly to rounding.  The GPU evaluates the same formulas in another
+arrangement (contracted with the cotangent first, FP64 rsqrt, other
+summation order): the VJP and the trajectory gradient are held to 1e-12 of
+the largest gradient component compared (measured: 1e-14 to 1e-16).  At the
+C4 size (N = 16384, 256 steps) the oracle is too slow, and the gradient is
+checked against central finite differences of GPU trajectories
+(h = 1e-6 relative), whose own truncation and rounding set the bound 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from pn_amd import lib as pn
+from pn_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = pn.Context(capacity=20000)
+    yield c
+    c.close()
+
+
+def all_coords(n):
+    out = [("m0",)]
+    for p in range(n):
+        for k in range(3):
+            out += [("x", p, k), ("v", p, k)]
+    return out
+
+
+def sample_coords(n, k, seed):
+    rng = np.random.default_rng(seed)
+    out = [("m0",), ("x", 0, 0), ("v", 0, 1)]
+    for _ in range(k):
+        out.append(("x" if rng.random() < 0.5 else "v", int(rng.integers(1, n)),
+                    int(rng.integers(0, 3))))
+    return out
+
+
+def pick(gx, gv, gm0, coords):
+    return np.array([gm0 if co[0] == "m0" else (gx if co[0] == "x" else gv)[co[1], co[2]]
+                     for co in coords])
+
+
+CASES = [  # (name, n_stars, m_bh, eps, c, coords)
+    ("plummer_n16_c1", 16, 0.05, 1e-2, 1.0, None),
+    ("plummer_n16_c30", 16, 0.05, 1e-3, 30.0, None),
+    ("plummer_n256_c100", 256, 0.02, 1e-3, 100.0, 48),
+    ("plummer_n1024_c5", 1024, 0.05, 1e-3, 5.0, 24),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_force_vjp_vs_complex_step(ctx, case):
+    name, ns, mbh, eps, c, k = case
+    x, v, m = pn.ic_plummer(ns, 11, a=1.0, m_bh=mbh)
+    u = np.random.default_rng(12).normal(size=x.shape)
+    ctx.set_params(eps, c, 2.0**-10, pn.FP64)
+    ctx.upload(x, v, m)
+    gx, gv, gm0 = ctx.force_vjp(u)
+    coords = all_coords(len(m)) if k is None else sample_coords(len(m), k, 13)
+    want = ob.cs_force_vjp(x, v, m, u, eps, c, coords)
+    got = pick(gx, gv, gm0, coords)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"{name}: max |g - g_cs| / max |g_cs| = {err:.3e}")
+    assert err <= 1e-12, err
+    # translation invariance: the position gradient sums to zero
+    assert np.abs(gx.sum(0)).max() <= 1e-12 * np.abs(gx).max() * len(m)
+
+
+def test_force_vjp_mode_independent(ctx):
+    """The adjoint is FP64 whatever the context's mode."""
+    x, v, m = pn.ic_plummer(300, 14, a=1.0, m_bh=0.02)
+    u = np.random.default_rng(15).normal(size=x.shape)
+    out = []
+    for mode in (pn.FP64, pn.MIXED):
+        ctx.set_params(1e-3, 50.0, 2.0**-10, mode)
+        ctx.upload(x, v, m)
+        out.append(ctx.force_vjp(u))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def run_adjoint(ctx, x, v, m, eps, c, dt, steps, interval, r_cut):
+    ctx.set_params(eps, c, dt, pn.FP64)
+    ctx.upload(x, v, m)
+    ctx.adjoint_forward(steps, interval)
+    xt, vt = ctx.download()
+    loss, lx, lv = workloads.radial_loss(xt, vt, r_cut)
+    gx0, gv0, gm0 = ctx.adjoint_backward(lx, lv)
+    return loss, (lx, lv), (gx0, gv0, gm0), (xt, vt)
+
+
+def test_adjoint_trajectory_vs_complex_step(ctx):
+    x, v, m = pn.ic_plummer(96, 21, a=1.0, m_bh=0.02)
+    eps, c, dt, steps = 1e-3, 100.0, 2.0**-10, 24
+    loss, lgrad, (gx0, gv0, gm0), (xt, vt) = run_adjoint(ctx, x, v, m, eps, c, dt, steps, 5,
+                                                          r_cut=0.6)
+    # the forward part is the stage driver itself
+    xo, vo = ob.orc_q1_steps(x, v, m, eps, c, dt, steps)
+    assert np.abs(xt - xo).max() <= 1e-12 and np.abs(vt - vo).max() <= 1e-11
+    coords = sample_coords(len(m), 20, 22)
+    want = ob.cs_trajectory_grad(x, v, m, eps, c, dt, steps, lgrad, coords)
+    got = pick(gx0, gv0, gm0, coords)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"trajectory n=97, {steps} steps: loss {loss:.6e}, rel err {err:.3e}")
+    assert err <= 1e-12, err
+
+
+def test_adjoint_checkpoint_interval_invariance(ctx):
+    """Recompute from checkpoints is exact: every interval gives the same bits."""
+    x, v, m = pn.ic_plummer(200, 23, a=1.0, m_bh=0.02)
+    res = [run_adjoint(ctx, x, v, m, 1e-3, 100.0, 2.0**-10, 12, k, 0.5)[2] for k in (1, 5, 12, 50)]
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            assert np.array_equal(np.asarray(a), np.asarray(b))
+
+
+def test_adjoint_leaves_initial_state(ctx):
+    x, v, m = pn.ic_plummer(64, 24, a=1.0, m_bh=0.02)
+    run_adjoint(ctx, x, v, m, 1e-3, 100.0, 2.0**-10, 7, 3, 0.5)
+    xd, vd = ctx.download()
+    assert np.array_equal(xd, x) and np.array_equal(vd, v)
+
+
+def test_adjoint_rejects_changed_state(ctx):
+    x, v, m = pn.ic_plummer(64, 25, a=1.0, m_bh=0.02)
+    ctx.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+    ctx.upload(x, v, m)
+    ctx.adjoint_forward(4, 2)
+    ctx.set_params(2e-3, 100.0, 2.0**-10, pn.FP64)
+    with pytest.raises(pn.Pn1Error, match="changed"):
+        ctx.adjoint_backward(np.zeros_like(x), np.zeros_like(v))
+
+
+def test_adjoint_c4_vs_finite_differences(ctx):
+    """The differentiation workload: N = 16384 Plummer (seed 8) + BH 0.02, c = 100,
+    eps = 1e-3, dt = 2^-10, 256 steps, checkpoints every 16, loss = mean
+    squared radial velocity within r < 0.2; 16 coordinates and m0 against
+    central differences of the same GPU trajectories."""
+    W = workloads.ADJOINT
+    x, v, m = pn.ic_plummer(W["n_stars"], W["seed"], a=1.0, m_bh=W["m_bh"])
+    loss, lgrad, (gx0, gv0, gm0), _ = run_adjoint(ctx, x, v, m, W["eps"], W["c"], W["dt"],
+                                                  W["steps"], W["interval"], W["r_cut"])
+    lx, lv = lgrad
+    assert loss > 0.0
+    # choose coordinates with non-negligible sensitivity: stars near the core
+    r = np.linalg.norm(x[1:] - x[0], axis=1)
+    rng = np.random.default_rng(26)
+    inner = 1 + np.flatnonzero(r < 0.3)
+    coords = [("m0",)] + [("x" if rng.random() < 0.5 else "v", int(p), int(rng.integers(0, 3)))
+                          for p in rng.choice(inner, 16, replace=False)]
+
+    def traj_loss(xx, vv, mm):
+        ctx.upload(xx, vv, mm)
+        ctx.step(W["steps"])
+        xt, vt = ctx.download()
+        return float(np.sum(lx * xt) + np.sum(lv * vt))     # linearised loss
+
+    h = 1e-6
+    got, want = [], []
+    for co in coords:
+        xx, vv, mm = x.copy(), v.copy(), m.copy()
+        arr = {"x": xx, "v": vv, "m0": mm}[co[0]]
+        idx = (0,) if co[0] == "m0" else (co[1], co[2])
+        step = h * max(1.0, abs(arr[idx]))
+        base = arr[idx]
+        arr[idx] = base + step
+        fp = traj_loss(xx, vv, mm)
+        arr[idx] = base - step
+        fm = traj_loss(xx, vv, mm)
+        want.append((fp - fm) / (2 * step))
+        got.append(gm0 if co[0] == "m0" else (gx0 if co[0] == "x" else gv0)[co[1], co[2]])
+    got, want = np.array(got), np.array(want)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"C4: loss {loss:.6e}, |g| max {np.abs(want).max():.3e}, FD rel err {err:.3e}")
+    assert err <= 1e-5, (got, want)
+
+
+def test_torch_autograd_wrapper(ctx):
+    """A loss written in torch, differentiated through the library's adjoint
+    (tests/torch_adjoint.py), against the numpy path and torch.autograd's
+    finite-difference gradcheck on a small system."""
+    import torch
+    import torch_adjoint as ta
+
+    x, v, m = pn.ic_plummer(12, 31, a=1.0, m_bh=0.02)
+    eps, c, dt, steps = 1e-2, 20.0, 2.0**-9, 10
+    ctx.set_params(eps, c, dt, pn.FP64)
+    x0 = torch.tensor(x, requires_grad=True)
+    v0 = torch.tensor(v, requires_grad=True)
+    m0 = torch.tensor(m[0], dtype=torch.float64, requires_grad=True)
+    xt, vt = ta.Trajectory.apply(x0, v0, m0, ctx, m, steps, 3)
+    loss = ta.radial_loss_torch(xt, vt, 0.8)
+    loss.backward()
+    # numpy path: same loss, gradient by hand, adjoint through the C ABI
+    _, _, (gx0, gv0, gm0), _ = run_adjoint(ctx, x, v, m, eps, c, dt, steps, 3, 0.8)
+    assert np.allclose(x0.grad.numpy(), gx0, rtol=1e-12, atol=1e-14)
+    assert np.allclose(v0.grad.numpy(), gv0, rtol=1e-12, atol=1e-14)
+    assert abs(m0.grad.item() - gm0) <= 1e-12 * max(1.0, abs(gm0))
+
+    def f(xx, vv, mm):
+        a, b = ta.Trajectory.apply(xx, vv, mm, ctx, m, steps, 3)
+        return ta.radial_loss_torch(a, b, 0.8)
+
+    assert torch.autograd.gradcheck(f, (x0.detach().requires_grad_(),
+                                        v0.detach().requires_grad_(),
+                                        m0.detach().requires_grad_()),
+                                    eps=1e-6, atol=1e-6, rtol=1e-4)
