@@ -22,7 +22,10 @@ The line also carries an "ensemble" object: the batched workload of 256
 seeded Plummer clusters (star counts log-uniform in [256, 8192]) advanced
 together through the ensemble API for --ens-steps steps (default 128), its
 interactions/s (sum of n^2 per force evaluation), pair-kernel roofline and
-mixed-vs-FP64 error, against the same clusters stepped one at a time.
+mixed-vs-FP64 error, against the same clusters stepped one at a time,
+and an "adjoint" object: the differentiation workload (N = 16384, 256
+steps, checkpoints every 16) forward and backward wall times and the force
+VJP's pair rate (FP64).
 """
 import argparse
 import json
@@ -213,6 +216,53 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False):
     }
 
 
+def run_adjoint(steps, interval, vjp_reps=5):
+    """The differentiation workload: N = 16384 Plummer (seed 8) + BH 0.02,
+    c = 100, eps = 1e-3, dt = 2^-10: `steps` steps forward with a checkpoint
+    every `interval`, the radial-velocity loss at r < 0.2, and the discrete
+    adjoint back to d loss / d (x0, v0, m0).  Also the force VJP alone
+    (pn1_force_vjp, two O(n^2) dual-number passes) against one FP64 force
+    evaluation at the same state."""
+    W = workloads.ADJOINT
+    x, v, m = pn.ic_plummer(W["n_stars"], W["seed"], a=1.0, m_bh=W["m_bh"])
+    n = len(m)
+    with pn.Context(capacity=n, device=0) as ctx:
+        ctx.set_params(W["eps"], W["c"], W["dt"], pn.FP64)
+        ctx.upload(x, v, m)
+        t0 = time.perf_counter()
+        ctx.adjoint_forward(steps, interval)
+        xt, vt = ctx.download()
+        t1 = time.perf_counter()
+        loss, lx, lv = workloads.radial_loss(xt, vt, W["r_cut"])
+        gx0, gv0, gm0 = ctx.adjoint_backward(lx, lv)
+        t2 = time.perf_counter()
+        u = np.random.default_rng(1).normal(size=x.shape)
+        ctx.force_vjp(u)
+        ctx.force()
+        ta = time.perf_counter()
+        for _ in range(vjp_reps):
+            ctx.force_vjp(u)
+        tb = time.perf_counter()
+        for _ in range(vjp_reps):
+            ctx.force()
+        tc = time.perf_counter()
+    vjp_ms = 1e3 * (tb - ta) / vjp_reps
+    force_ms = 1e3 * (tc - tb) / vjp_reps
+    return {
+        "workload": f"Plummer N={W['n_stars']} (seed {W['seed']}) + BH m={W['m_bh']}, "
+                    f"c={W['c']:g}, eps={W['eps']:g}, dt=2^-10, {steps} steps, checkpoint every "
+                    f"{interval}; loss: mean squared radial velocity within r < {W['r_cut']}",
+        "forward_s": t1 - t0, "backward_s": t2 - t1,
+        "backward_over_forward": (t2 - t1) / (t1 - t0),
+        "loss": loss, "grad_m0": gm0,
+        "grad_x0_max": float(np.abs(gx0).max()), "grad_v0_max": float(np.abs(gv0).max()),
+        "vjp_ms": vjp_ms, "force_ms": force_ms,
+        "vjp_pairs_per_s": float(n) * n / (vjp_ms * 1e-3),
+        "vjp_over_force_time": vjp_ms / force_ms,
+        "dtype": "f64",
+    }
+
+
 def roofline_flops(inter, avg_ms, peak, traffic=None):
     achieved = FLOPS_PER_INTERACTION * inter / (avg_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak / 1e12,
@@ -231,6 +281,8 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=128,
                     help="steps of the batched-ensemble workload (0 skips it)")
     ap.add_argument("--ens-chunk", type=int, default=0)
+    ap.add_argument("--adj-steps", type=int, default=256,
+                    help="steps of the differentiation workload (0 skips it)")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel of the ensemble leg directly instead of "
                          "replaying captured step graphs (for profilers)")
@@ -298,6 +350,8 @@ def main():
     if args.ens_steps > 0:
         out["ensemble"] = run_ensemble(args.ens_steps, args.warmup, chunk=args.ens_chunk,
                                        tile=args.tile, variant=args.variant, eager=args.eager)
+    if args.adj_steps > 0:
+        out["adjoint"] = run_adjoint(args.adj_steps, min(16, args.adj_steps))
     if not args.no_cpu:
         old = threading.stack_size(256 << 20)
         res = {}
