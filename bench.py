@@ -22,7 +22,8 @@ The line also carries an "ensemble" object: the batched workload of 256
 seeded Plummer clusters (star counts log-uniform in [256, 8192]) advanced
 together through the ensemble API for --ens-steps steps (default 128), its
 interactions/s (sum of n^2 per force evaluation), pair-kernel roofline and
-mixed-vs-FP64 error, against the same clusters stepped one at a time,
+mixed-vs-FP64 error, against the same clusters stepped one at a time, and
+the adjoint of every cluster's BH-mass gradient over the same steps,
 and an "adjoint" object: the differentiation workload (N = 16384, 256
 steps, checkpoints every 16) forward and backward wall times and the force
 VJP's pair rate (FP64).
@@ -147,7 +148,7 @@ def time_steps(obj, steps, warmup, eager=False):
     return wall
 
 
-def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False):
+def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_steps=0):
     """The batched workload: 256 seeded Plummer clusters (star counts
     log-uniform in [256, 8192], BH masses uniform in [0.005, 0.05], c = 100,
     eps = 1e-3, dt = 2^-10), all advanced by one pn1_ens_step per step (one
@@ -183,6 +184,22 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False):
         ens.set_params(W["eps"], W["c"], W["dt"], pn.MIXED)
         a32 = np.concatenate(ens.force())
         chunk_used, tile_used, var_used = ens.get_tuning()
+        # the BH-mass gradient of every cluster's radial loss (FP64 adjoint)
+        adj = None
+        if adj_steps > 0:
+            ens.upload(cl)
+            t0 = time.perf_counter()
+            ens.adjoint_forward(adj_steps, 16)
+            xs, vs = ens.download()
+            t1 = time.perf_counter()
+            lg = [workloads.radial_loss(x, v, 0.2) for x, v in zip(xs, vs)]
+            _, _, gm0 = ens.adjoint_backward([g[1] for g in lg], [g[2] for g in lg])
+            t2 = time.perf_counter()
+            adj = {"steps": adj_steps, "checkpoint_interval": 16,
+                   "loss": "per cluster: mean squared radial velocity within r < 0.2 of its BH",
+                   "forward_s": t1 - t0, "backward_s": t2 - t1,
+                   "grad_m0_median_abs": float(np.median(np.abs(gm0))),
+                   "grad_m0_max_abs": float(np.abs(gm0).max())}
     err = rel_err(a32, a64)
     sep_wall = 0.0
     with pn.Context(capacity=max(len(m) for _, _, m in cl)) as ctx:
@@ -213,6 +230,7 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False):
                                    "energy_drift_max": mixed["energy_drift_max"],
                                    "energy_drift_median": mixed["energy_drift_median"]},
         "tuning": {"chunk": chunk_used, "tile": tile_used, "variant": var_used},
+        "adjoint_bh_mass": adj,
     }
 
 
@@ -349,7 +367,8 @@ def main():
     }
     if args.ens_steps > 0:
         out["ensemble"] = run_ensemble(args.ens_steps, args.warmup, chunk=args.ens_chunk,
-                                       tile=args.tile, variant=args.variant, eager=args.eager)
+                                       tile=args.tile, variant=args.variant, eager=args.eager,
+                                       adj_steps=args.ens_steps if args.adj_steps > 0 else 0)
     if args.adj_steps > 0:
         out["adjoint"] = run_adjoint(args.adj_steps, min(16, args.adj_steps))
     if not args.no_cpu:

@@ -190,6 +190,16 @@ const char *pn1_ens_last_error(const pn1_ens *ens);
 int  pn1_ens_set_tuning(pn1_ens *ens, int chunk, int tile, int variant);
 int  pn1_ens_get_tuning(const pn1_ens *ens, int *chunk, int *tile,
                         int *variant);
+/* Adjoint, per cluster, as pn1_force_vjp / pn1_adjoint_forward /
+ * pn1_adjoint_backward (FP64 whatever the mode); gm0 (may be NULL) gets
+ * one value per cluster: the derivative with respect to that cluster's BH
+ * mass.  No counterpart in the reference. */
+int  pn1_ens_force_vjp(pn1_ens *ens, const double *u, double *gx,
+                       double *gv, double *gm0);
+int  pn1_ens_adjoint_forward(pn1_ens *ens, int steps, int interval);
+int  pn1_ens_adjoint_backward(pn1_ens *ens, const double *gx,
+                              const double *gv, double *gx0, double *gv0,
+                              double *gm0);
 /* Pair-kernel timing, as pn1_profile_enable / pn1_profile_read. */
 int  pn1_ens_profile_enable(pn1_ens *ens, int on);
 int  pn1_ens_profile_read(pn1_ens *ens, int *launches, double *total_ms);

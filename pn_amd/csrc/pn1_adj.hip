@@ -28,6 +28,7 @@
 
 #include "pn1_adj_kernels.hpp"
 #include "pn1_ctx.hpp"
+#include "pn1_ens.hpp"
 
 using namespace pn1dev;
 using namespace pn1host;
@@ -292,6 +293,416 @@ int pn1_adjoint_backward(pn1_ctx* c, const double* gx, const double* gv, double*
         *gm0 = h[0];
     }
     return PN1_OK;
+}
+
+}  // extern "C"
+
+// =========================================================================
+// Ensembles (pn1_ens_force_vjp, pn1_ens_adjoint_*): the same VJP and stage
+// adjoint for every cluster at once.  The two pair passes are persistent
+// launches over the ensemble's own work items (cluster, 256-star block,
+// j chunk) and per-XCD queues; each item writes its own partial slots.
+// =========================================================================
+namespace pn1dev {
+
+__global__ __launch_bounds__(WG) void k_ens_adj_prep(const double* __restrict__ y,
+                                                    const double* __restrict__ mu,
+                                                    const double* __restrict__ m,
+                                                    const int* __restrict__ coff,
+                                                    const EBlk* __restrict__ blk, Phys P,
+                                                    ARec* __restrict__ rec)
+{
+    const EBlk b = blk[blockIdx.x];
+    const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
+    const int li = b.first + (int)threadIdx.x;
+    if (li < nst) adj_prep_star(y, mu, m, (size_t)P.cap, P.eps2, off + 1 + li, off, rec);
+}
+
+__global__ void k_ens_adj_reset(unsigned int* __restrict__ counter)
+{
+    if (threadIdx.x < NXCD) counter[threadIdx.x] = 0u;
+}
+
+template <int PASS>
+__global__ __launch_bounds__(WG) void k_ens_adj_pair(const ARec* __restrict__ rec,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ mu,
+                                                    const double* __restrict__ m,
+                                                    double* __restrict__ part,
+                                                    const EItem* __restrict__ items,
+                                                    unsigned int* __restrict__ counter,
+                                                    const int* __restrict__ qoff, Phys P)
+{
+    int q = 0;
+    for (;;) {
+        const int it = ens_take(counter, qoff, q);
+        if (it < 0) break;
+        const EItem w = items[it];
+        const int li = w.ib * WG + (int)threadIdx.x;
+        const int i = w.off + 1 + (li < w.nst ? li : w.nst - 1);
+        const ABH B = load_abh(y, mu, m, P.cap, w.off);
+        const ARec me = rec[i];
+        double* pp = part + w.pslot * NADJ + li;
+        if (PASS == 1) {
+            double acc[NADJ1];
+            adj_pass1(rec, B, me, i, w.j0, w.j1, P.eps2, P.lam, acc);
+            if (li < w.nst) {
+#pragma unroll
+                for (int k = 0; k < NADJ1; ++k) pp[(size_t)k * w.nst] = acc[k];
+            }
+        } else {
+            double acc[NADJ2];
+            adj_pass2(rec, B, me, i, w.j0, w.j1, P.eps2, P.lam, acc);
+            if (li < w.nst) {
+#pragma unroll
+                for (int k = 0; k < NADJ2; ++k) pp[(size_t)(NADJ1 + k) * w.nst] = acc[k];
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(WG) void k_ens_adj_fin(const ARec* __restrict__ rec,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ mu,
+                                                   const double* __restrict__ m,
+                                                   const int* __restrict__ coff,
+                                                   const EBlk* __restrict__ blk,
+                                                   const double* __restrict__ part,
+                                                   const long long* __restrict__ poff,
+                                                   double* __restrict__ out,
+                                                   double* __restrict__ ablk, Phys P, int L)
+{
+    const EBlk b = blk[blockIdx.x];
+    const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
+    const int li = b.first + (int)threadIdx.x;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    if (li < nst) {
+        const int p = off + 1 + li;
+        const ABH B = load_abh(y, mu, m, P.cap, off);
+        double g[NADJ1];
+        adj_star_grad(rec[p], B, P.eps, P.eps2, P.lam, g);
+        const double* base = part + poff[b.c] * NADJ + li;
+        const int S = ens_nchunks(nst, L);
+        for (int s = 0; s < S; ++s) {
+#pragma unroll
+            for (int k = 0; k < NADJ1; ++k) g[k] += base[(size_t)(s * NADJ + k) * nst];
+#pragma unroll
+            for (int k = 0; k < NADJ2; ++k) g[k] += base[(size_t)(s * NADJ + NADJ1 + k) * nst];
+        }
+        adj_store(g, mu, out, (size_t)P.cap, p, t);
+    }
+    block_sum<4>(t, ablk + (size_t)blockIdx.x * 4);
+}
+
+// One workgroup per cluster: its BH's rows and its m0 derivative
+// (gm0[2c + 1] this evaluation, added to gm0[2c]).
+__global__ __launch_bounds__(WG) void k_ens_adj_bh(const double* __restrict__ ablk,
+                                                  const int* __restrict__ cblk,
+                                                  const int* __restrict__ coff,
+                                                  const double* __restrict__ mu,
+                                                  double* __restrict__ out,
+                                                  double* __restrict__ gm0, Phys P)
+{
+    const int c = blockIdx.x;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = cblk[c] + (int)threadIdx.x; b < cblk[c + 1]; b += WG)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s[k] += ablk[(size_t)b * 4 + k];
+    __shared__ double tot[4];
+    block_sum<4>(s, tot);
+    if (threadIdx.x == 0) {
+        const size_t cap = (size_t)P.cap;
+        const int b0 = coff[c];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            out[(size_t)k * cap + b0] = -tot[k];
+            out[(size_t)(3 + k) * cap + b0] = mu[(size_t)k * cap + b0];
+        }
+        gm0[2 * c + 1] = tot[3];
+        gm0[2 * c] += tot[3];
+    }
+}
+
+}  // namespace pn1dev
+
+struct pn1_ens_adj {
+    double *mu = nullptr, *out = nullptr, *L = nullptr, *LY1 = nullptr, *LY2 = nullptr;
+    double *T = nullptr, *ablk = nullptr, *gm0 = nullptr;
+    ARec* rec = nullptr;
+    double* ckpt = nullptr;
+    double* seg = nullptr;
+    int nck_cap = 0, seg_cap = 0, ablk_cap = 0;
+    int grid1 = 0, grid2 = 0;
+    bool have_fwd = false;
+    int steps = 0, interval = 0;
+    unsigned gen = 0;
+};
+
+namespace pn1host {
+
+void ens_adj_free(pn1_ens* e)
+{
+    pn1_ens_adj* a = e ? e->adj : nullptr;
+    if (!a) return;
+    void* bufs[] = {a->mu, a->out, a->L, a->LY1, a->LY2, a->T, a->ablk, a->gm0, a->rec,
+                    a->ckpt, a->seg};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    delete a;
+    e->adj = nullptr;
+}
+
+}  // namespace pn1host
+
+namespace {
+
+template <typename T>
+int ealloc_adj(pn1_ens* e, T** p, size_t count)
+{
+    if (hipMalloc((void**)p, (count ? count : 1) * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return eerr(e, PN1_ENOMEM, "hipMalloc of %zu bytes (adjoint) failed", count * sizeof(T));
+    }
+    return PN1_OK;
+}
+
+template <typename K>
+int adj_resident_grid(pn1_ens* e, K kernel, int* out)
+{
+    int per_cu = 0, dev = 0;
+    hipDeviceProp_t prop;
+    EHIPCHK(e, hipGetDevice(&dev));
+    EHIPCHK(e, hipGetDeviceProperties(&prop, dev));
+    EHIPCHK(e, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, WG, 0));
+    *out = (per_cu > 0 ? per_cu : 1) * prop.multiProcessorCount;
+    return PN1_OK;
+}
+
+int ens_adj_get(pn1_ens* e, pn1_ens_adj** out)
+{
+    if (!e->adj) {
+        pn1_ens_adj* a = new pn1_ens_adj;
+        e->adj = a;
+        const size_t s6 = 6 * (size_t)e->cap;
+        int rc = PN1_OK;
+        if (!rc) rc = ealloc_adj(e, &a->mu, s6);
+        if (!rc) rc = ealloc_adj(e, &a->out, s6);
+        if (!rc) rc = ealloc_adj(e, &a->L, s6);
+        if (!rc) rc = ealloc_adj(e, &a->LY1, s6);
+        if (!rc) rc = ealloc_adj(e, &a->LY2, s6);
+        if (!rc) rc = ealloc_adj(e, &a->T, s6);
+        if (!rc) rc = ealloc_adj(e, &a->gm0, 2 * (size_t)e->capc);
+        if (!rc) rc = ealloc_adj(e, &a->rec, (size_t)e->cap + 2);
+        if (!rc) rc = adj_resident_grid(e, k_ens_adj_pair<1>, &a->grid1);
+        if (!rc) rc = adj_resident_grid(e, k_ens_adj_pair<2>, &a->grid2);
+        if (rc) {
+            ens_adj_free(e);
+            return rc;
+        }
+    }
+    pn1_ens_adj* a = e->adj;
+    if (e->nb > a->ablk_cap) {          // per-block partials follow the block table
+        if (a->ablk) (void)hipFree(a->ablk);
+        a->ablk = nullptr;
+        a->ablk_cap = 0;
+        int rc = ealloc_adj(e, &a->ablk, (size_t)e->nb * 4);
+        if (rc) return rc;
+        a->ablk_cap = e->nb;
+    }
+    *out = a;
+    return PN1_OK;
+}
+
+int ens_enqueue_vjp(pn1_ens* e, pn1_ens_adj* a, const double* y, const double* mu,
+                    double* out, const Phys& P)
+{
+    if (e->nb)
+        hipLaunchKernelGGL(k_ens_adj_prep, dim3(e->nb), dim3(WG), 0, e->st, y, mu, e->m, e->coff,
+                           e->blk, P, a->rec);
+    if (e->nitems) {
+        double* part = (double*)e->part;
+        hipLaunchKernelGGL(k_ens_adj_reset, dim3(1), dim3(64), 0, e->st, e->counter);
+        hipLaunchKernelGGL(k_ens_adj_pair<1>, dim3(std::min(a->grid1, e->nitems)), dim3(WG), 0,
+                           e->st, a->rec, y, mu, e->m, part, e->items, e->counter, e->qoff, P);
+        hipLaunchKernelGGL(k_ens_adj_reset, dim3(1), dim3(64), 0, e->st, e->counter);
+        hipLaunchKernelGGL(k_ens_adj_pair<2>, dim3(std::min(a->grid2, e->nitems)), dim3(WG), 0,
+                           e->st, a->rec, y, mu, e->m, part, e->items, e->counter, e->qoff, P);
+    }
+    if (e->nb)
+        hipLaunchKernelGGL(k_ens_adj_fin, dim3(e->nb), dim3(WG), 0, e->st, a->rec, y, mu, e->m,
+                           e->coff, e->blk, (const double*)e->part, e->poff, out, a->ablk, P,
+                           e->L);
+    hipLaunchKernelGGL(k_ens_adj_bh, dim3(e->nc), dim3(WG), 0, e->st, a->ablk, e->cblk, e->coff,
+                       mu, out, a->gm0, P);
+    EHIPCHK(e, hipGetLastError());
+    return PN1_OK;
+}
+
+void ens_lin(pn1_ens* e, double* out, double ca, const double* A, double cb = 0.0,
+             const double* B = nullptr, double cc = 0.0, const double* C = nullptr,
+             double cd = 0.0, const double* D = nullptr)
+{
+    hipLaunchKernelGGL(k_adj_lin, dim3((e->n + WG - 1) / WG), dim3(WG), 0, e->st, out, e->n,
+                       e->cap, ca, A, cb, B, cc, C, cd, D);
+}
+
+int ens_put6(pn1_ens* e, const double* hx, const double* hv, double* dst)
+{
+    const size_t cap = (size_t)e->cap;
+    double* h = e->h_stage;
+    for (int i = 0; i < e->n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            h[k * cap + i] = hx ? hx[3 * i + k] : 0.0;
+            h[(3 + k) * cap + i] = hv ? hv[3 * i + k] : 0.0;
+        }
+    for (int k = 0; k < 6; ++k)
+        EHIPCHK(e, hipMemcpyAsync(dst + k * cap, h + k * cap, e->n * sizeof(double),
+                                  hipMemcpyHostToDevice, e->st));
+    EHIPCHK(e, hipStreamSynchronize(e->st));
+    return PN1_OK;
+}
+
+int ens_copy6(pn1_ens* e, double* dst, const double* src)
+{
+    const size_t cap = (size_t)e->cap;
+    for (int k = 0; k < 6; ++k)
+        EHIPCHK(e, hipMemcpyAsync(dst + k * cap, src + k * cap, e->n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, e->st));
+    return PN1_OK;
+}
+
+int ens_get_gm0(pn1_ens* e, pn1_ens_adj* a, int which, double* gm0)
+{
+    std::vector<double> h(2 * (size_t)e->nc);
+    EHIPCHK(e, hipMemcpyAsync(h.data(), a->gm0, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              e->st));
+    EHIPCHK(e, hipStreamSynchronize(e->st));
+    for (int c = 0; c < e->nc; ++c) gm0[c] = h[2 * (size_t)c + which];
+    return PN1_OK;
+}
+
+struct EnsFp64Scope {
+    pn1_ens* e;
+    int mode;
+    explicit EnsFp64Scope(pn1_ens* ee) : e(ee), mode(ee->mode) { e->mode = PN1_FP64; }
+    ~EnsFp64Scope() { e->mode = mode; }
+};
+
+int ens_stages(pn1_ens* e, int last, const Phys& P)
+{
+    for (int stage = 1; stage <= last; ++stage) {
+        int rc = ens_enqueue_force(e, stage, P);
+        if (rc) return rc;
+    }
+    return PN1_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pn1_ens_force_vjp(pn1_ens* e, const double* u, double* gx, double* gv, double* gm0)
+{
+    int rc;
+    if (!ens_ready(e, &rc)) return rc;
+    if (!u || !gx || !gv) return eerr(e, PN1_EINVAL, "null argument");
+    pn1_ens_adj* a;
+    if ((rc = ens_adj_get(e, &a))) return rc;
+    const Phys P = ens_phys(e);
+    if ((rc = ens_put6(e, nullptr, u, a->mu))) return rc;
+    EHIPCHK(e, hipMemsetAsync(a->gm0, 0, 2 * (size_t)e->nc * sizeof(double), e->st));
+    if ((rc = ens_enqueue_vjp(e, a, e->X, a->mu, a->out, P))) return rc;
+    const size_t cap = (size_t)e->cap;
+    if ((rc = ens_fetch3(e, a->out, gx))) return rc;
+    if ((rc = ens_fetch3(e, a->out + 3 * cap, gv))) return rc;
+    return gm0 ? ens_get_gm0(e, a, 1, gm0) : PN1_OK;
+}
+
+int pn1_ens_adjoint_forward(pn1_ens* e, int steps, int interval)
+{
+    int rc;
+    if (!ens_ready(e, &rc)) return rc;
+    if (steps < 1 || interval < 1)
+        return eerr(e, PN1_EINVAL, "steps (%d) and interval (%d) must be >= 1", steps, interval);
+    pn1_ens_adj* a;
+    if ((rc = ens_adj_get(e, &a))) return rc;
+    const int nck = (steps + interval - 1) / interval;
+    const size_t s6 = 6 * (size_t)e->cap;
+    if (nck > a->nck_cap) {
+        if (a->ckpt) (void)hipFree(a->ckpt);
+        a->ckpt = nullptr;
+        a->nck_cap = 0;
+        if ((rc = ealloc_adj(e, &a->ckpt, (size_t)nck * s6))) return rc;
+        a->nck_cap = nck;
+    }
+    const int seg = std::min(interval, steps);
+    if (seg > a->seg_cap) {
+        if (a->seg) (void)hipFree(a->seg);
+        a->seg = nullptr;
+        a->seg_cap = 0;
+        if ((rc = ealloc_adj(e, &a->seg, (size_t)seg * s6))) return rc;
+        a->seg_cap = seg;
+    }
+    EnsFp64Scope fp64(e);
+    edrop_graph(e);
+    const Phys P = ens_phys(e);
+    for (int t = 0; t < steps; ++t) {
+        if (t % interval == 0 && (rc = ens_copy6(e, a->ckpt + (size_t)(t / interval) * s6, e->X)))
+            return rc;
+        if ((rc = ens_stages(e, 3, P))) return rc;
+    }
+    EHIPCHK(e, hipStreamSynchronize(e->st));
+    a->have_fwd = true;
+    a->steps = steps;
+    a->interval = interval;
+    a->gen = e->gen;
+    return PN1_OK;
+}
+
+int pn1_ens_adjoint_backward(pn1_ens* e, const double* gx, const double* gv, double* gx0,
+                             double* gv0, double* gm0)
+{
+    int rc;
+    if (!ens_ready(e, &rc)) return rc;
+    if (!gx || !gv || !gx0 || !gv0) return eerr(e, PN1_EINVAL, "null argument");
+    pn1_ens_adj* a = e->adj;
+    if (!a || !a->have_fwd)
+        return eerr(e, PN1_EINVAL, "no pn1_ens_adjoint_forward run to reverse");
+    if (a->gen != e->gen)
+        return eerr(e, PN1_EINVAL, "state or parameters changed since pn1_ens_adjoint_forward");
+    if ((rc = ens_adj_get(e, &a))) return rc;
+    EnsFp64Scope fp64(e);
+    edrop_graph(e);
+    const Phys P = ens_phys(e);
+    const double dt = e->dt;
+    const size_t s6 = 6 * (size_t)e->cap;
+    if ((rc = ens_put6(e, gx, gv, a->L))) return rc;
+    EHIPCHK(e, hipMemsetAsync(a->gm0, 0, 2 * (size_t)e->nc * sizeof(double), e->st));
+    const int nck = (a->steps + a->interval - 1) / a->interval;
+    for (int k = nck - 1; k >= 0; --k) {
+        const int t0 = k * a->interval, t1 = std::min(a->steps, t0 + a->interval);
+        if ((rc = ens_copy6(e, e->X, a->ckpt + (size_t)k * s6))) return rc;
+        for (int t = t0; t < t1; ++t) {
+            if ((rc = ens_copy6(e, a->seg + (size_t)(t - t0) * s6, e->X))) return rc;
+            if (t + 1 < t1 && (rc = ens_stages(e, 3, P))) return rc;
+        }
+        for (int t = t1 - 1; t >= t0; --t) {
+            if ((rc = ens_copy6(e, e->X, a->seg + (size_t)(t - t0) * s6))) return rc;
+            if ((rc = ens_stages(e, 2, P))) return rc;
+            ens_lin(e, a->mu, dt / 6.0, a->L);
+            if ((rc = ens_enqueue_vjp(e, a, e->H2, a->mu, a->LY2, P))) return rc;
+            ens_lin(e, a->mu, dt / 6.0, a->L, dt / 2.0, a->LY2);
+            if ((rc = ens_enqueue_vjp(e, a, e->H1, a->mu, a->LY1, P))) return rc;
+            ens_lin(e, a->mu, dt / 6.0, a->L, dt / 2.0, a->LY1);
+            if ((rc = ens_enqueue_vjp(e, a, e->X, a->mu, a->T, P))) return rc;
+            ens_lin(e, a->L, 1.0, a->L, 1.0, a->LY2, 1.0, a->LY1, 1.0, a->T);
+        }
+    }
+    EHIPCHK(e, hipGetLastError());
+    const size_t cap = (size_t)e->cap;
+    if ((rc = ens_fetch3(e, a->L, gx0))) return rc;
+    if ((rc = ens_fetch3(e, a->L + 3 * cap, gv0))) return rc;
+    return gm0 ? ens_get_gm0(e, a, 0, gm0) : PN1_OK;
 }
 
 }  // extern "C"

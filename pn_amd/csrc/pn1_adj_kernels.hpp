@@ -49,17 +49,18 @@ struct ABH {
     double x0[3], u0[3], m0;
 };
 
+// The BH of the cluster whose BH sits at index b0.
 __device__ __forceinline__ ABH load_abh(const double* __restrict__ st,
                                         const double* __restrict__ mu,
-                                        const double* __restrict__ m, int cap)
+                                        const double* __restrict__ m, int cap, int b0)
 {
     ABH b;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        b.x0[k] = st[(size_t)k * cap];
-        b.u0[k] = mu[(size_t)(3 + k) * cap];
+        b.x0[k] = st[(size_t)k * cap + b0];
+        b.u0[k] = mu[(size_t)(3 + k) * cap + b0];
     }
-    b.m0 = m[0];
+    b.m0 = m[b0];
     return b;
 }
 
@@ -170,25 +171,136 @@ __device__ __forceinline__ auto adj_star_psi(const T (&vi)[3], const T (&di)[3],
     return newton + lam * (t3 + t4 - t5 + t6 + t7 - t8);
 }
 
-// Records of the state y (SoA, stride cap) with cotangent u = mu_v.
+// Record of star p (BH at b0) of the state y (SoA, stride cap) with
+// cotangent u = mu_v.
+__device__ __forceinline__ void adj_prep_star(const double* __restrict__ y,
+                                              const double* __restrict__ mu,
+                                              const double* __restrict__ m, size_t cap,
+                                              double eps2, int p, int b0,
+                                              ARec* __restrict__ rec)
+{
+    ARec r;
+    r.x = y[p]; r.y = y[cap + p]; r.z = y[2 * cap + p];
+    r.vx = y[3 * cap + p]; r.vy = y[4 * cap + p]; r.vz = y[5 * cap + p];
+    r.ux = mu[3 * cap + p]; r.uy = mu[4 * cap + p]; r.uz = mu[5 * cap + p];
+    r.m = m[p];
+    r.dx = r.x - y[b0]; r.dy = r.y - y[cap + b0]; r.dz = r.z - y[2 * cap + b0];
+    r.D2 = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
+    r.e = rsq64(r.D2 + eps2);
+    r.pad = 0.0;
+    rec[p] = r;
+}
+
 __global__ __launch_bounds__(WG) void k_adj_prep(const double* __restrict__ y,
                                                 const double* __restrict__ mu,
                                                 const double* __restrict__ m, Phys P,
                                                 ARec* __restrict__ rec)
 {
     const int p = blockIdx.x * WG + threadIdx.x + 1;
-    if (p >= P.n) return;
-    const size_t cap = (size_t)P.cap;
-    ARec r;
-    r.x = y[p]; r.y = y[cap + p]; r.z = y[2 * cap + p];
-    r.vx = y[3 * cap + p]; r.vy = y[4 * cap + p]; r.vz = y[5 * cap + p];
-    r.ux = mu[3 * cap + p]; r.uy = mu[4 * cap + p]; r.uz = mu[5 * cap + p];
-    r.m = m[p];
-    r.dx = r.x - y[0]; r.dy = r.y - y[cap]; r.dz = r.z - y[2 * cap];
-    r.D2 = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
-    r.e = rsq64(r.D2 + P.eps2);
-    r.pad = 0.0;
-    rec[p] = r;
+    if (p < P.n) adj_prep_star(y, mu, m, (size_t)P.cap, P.eps2, p, 0, rec);
+}
+
+// Pass 1 over the records [j0, j1) for the lane's star i (j = i skipped):
+// d/d(x_i, v_i, m0) of sum_j phi_ij.
+__device__ __forceinline__ void adj_pass1(const ARec* __restrict__ rec, const ABH& B,
+                                          const ARec& me, int i, int j0, int j1, double eps2,
+                                          double lam, double (&acc)[NADJ1])
+{
+    using D = Dual<NADJ1>;
+    const D xi[3] = {dseed<NADJ1>(me.x, 0), dseed<NADJ1>(me.y, 1), dseed<NADJ1>(me.z, 2)};
+    const D vi[3] = {dseed<NADJ1>(me.vx, 3), dseed<NADJ1>(me.vy, 4), dseed<NADJ1>(me.vz, 5)};
+    const D m0 = dseed<NADJ1>(B.m0, 6);
+    const D im0 = dinv(m0);
+    const D di[3] = {xi[0] - B.x0[0], xi[1] - B.x0[1], xi[2] - B.x0[2]};
+    const D rb2 = dot(di, di);
+    const D rbe = drsq(rb2 + eps2);
+    const D rbe3 = rbe * rbe * rbe;
+    const D vvi = dot(vi, vi);
+    const double ume[3] = {me.ux, me.uy, me.uz};
+#pragma unroll
+    for (int k = 0; k < NADJ1; ++k) acc[k] = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        const ARec J = rec[j];
+        const double xj[3] = {J.x, J.y, J.z}, vj[3] = {J.vx, J.vy, J.vz};
+        const double dj[3] = {J.dx, J.dy, J.dz};
+        const double rce3 = J.e * J.e * J.e;
+        const D phi = adj_pair_phi(xi, vi, di, rb2, rbe, rbe3, vvi, xj, vj, dj, J.D2, J.e, rce3,
+                                   me.m, J.m, m0, im0, ume, B.u0, eps2, lam);
+        if (j != i) {
+#pragma unroll
+            for (int k = 0; k < NADJ1; ++k) acc[k] += phi.d[k];
+        }
+    }
+}
+
+// Pass 2 over the records [i0, i1) on the i side, for the lane's star j
+// (i = j skipped): d/d(x_j, v_j) of sum_i phi_ij.
+__device__ __forceinline__ void adj_pass2(const ARec* __restrict__ rec, const ABH& B,
+                                          const ARec& me, int jme, int i0, int i1, double eps2,
+                                          double lam, double (&acc)[NADJ2])
+{
+    using D = Dual<NADJ2>;
+    const D xj[3] = {dseed<NADJ2>(me.x, 0), dseed<NADJ2>(me.y, 1), dseed<NADJ2>(me.z, 2)};
+    const D vj[3] = {dseed<NADJ2>(me.vx, 3), dseed<NADJ2>(me.vy, 4), dseed<NADJ2>(me.vz, 5)};
+    const D dj[3] = {xj[0] - B.x0[0], xj[1] - B.x0[1], xj[2] - B.x0[2]};
+    const D rc2 = dot(dj, dj);
+    const D rce = drsq(rc2 + eps2);
+    const D rce3 = rce * rce * rce;
+    const double im0 = 1.0 / B.m0;
+#pragma unroll
+    for (int k = 0; k < NADJ2; ++k) acc[k] = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const ARec I = rec[i];
+        const double xi[3] = {I.x, I.y, I.z}, vi[3] = {I.vx, I.vy, I.vz};
+        const double di[3] = {I.dx, I.dy, I.dz};
+        const double ui[3] = {I.ux, I.uy, I.uz};
+        const double rbe3 = I.e * I.e * I.e;
+        const double vvi = I.vx * I.vx + I.vy * I.vy + I.vz * I.vz;
+        const D phi = adj_pair_phi(xi, vi, di, I.D2, I.e, rbe3, vvi, xj, vj, dj, rc2, rce, rce3,
+                                   I.m, me.m, B.m0, im0, ui, B.u0, eps2, lam);
+        if (i != jme) {
+#pragma unroll
+            for (int k = 0; k < NADJ2; ++k) acc[k] += phi.d[k];
+        }
+    }
+}
+
+// d (psi_i + phi_ii) / d(x_i, v_i, m0) for star i with record `me`.  The
+// self pair phi_ii is taken with both sides on the same variables: r = 0
+// and its tangents cancel exactly, as in the reference's own sum.
+__device__ __forceinline__ void adj_star_grad(const ARec& me, const ABH& B, double eps,
+                                              double eps2, double lam, double (&g)[NADJ1])
+{
+    using D = Dual<NADJ1>;
+    const D xi[3] = {dseed<NADJ1>(me.x, 0), dseed<NADJ1>(me.y, 1), dseed<NADJ1>(me.z, 2)};
+    const D vi[3] = {dseed<NADJ1>(me.vx, 3), dseed<NADJ1>(me.vy, 4), dseed<NADJ1>(me.vz, 5)};
+    const D m0 = dseed<NADJ1>(B.m0, 6);
+    const D im0 = dinv(m0);
+    const D di[3] = {xi[0] - B.x0[0], xi[1] - B.x0[1], xi[2] - B.x0[2]};
+    const D rb2 = dot(di, di);
+    const D rbe = drsq(rb2 + eps2);
+    const double ume[3] = {me.ux, me.uy, me.uz};
+    const D psi = adj_star_psi(vi, di, rb2, rbe, me.m, m0, im0, ume, B.u0, eps, lam);
+    const D rbe3 = rbe * rbe * rbe;
+    const D vvi = dot(vi, vi);
+    const D self = adj_pair_phi(xi, vi, di, rb2, rbe, rbe3, vvi, xi, vi, di, rb2, rbe, rbe3,
+                                me.m, me.m, m0, im0, ume, B.u0, eps2, lam);
+#pragma unroll
+    for (int k = 0; k < NADJ1; ++k) g[k] = psi.d[k] + self.d[k];
+}
+
+// out_x = g_x, out_v = mu_x + g_v for particle p; t = {g_x, d/dm0}.
+__device__ __forceinline__ void adj_store(const double (&g)[NADJ1], const double* __restrict__ mu,
+                                          double* __restrict__ out, size_t cap, int p,
+                                          double (&t)[4])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        out[(size_t)k * cap + p] = g[k];
+        out[(size_t)(3 + k) * cap + p] = mu[(size_t)k * cap + p] + g[3 + k];
+        t[k] = g[k];
+    }
+    t[3] = g[6];
 }
 
 // Pair passes.  Partials: part[(s * NADJ + row) * cap + p], rows 0..6 from
@@ -207,66 +319,19 @@ __global__ __launch_bounds__(WG) void k_adj_pair(const ARec* __restrict__ rec,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const ABH B = load_abh(y, mu, m, P.cap);
+    const ABH B = load_abh(y, mu, m, P.cap, 0);
     const JRange R = j_range(s, S, n, ib);
     const ARec me = rec[i];
-    const double ume[3] = {me.ux, me.uy, me.uz};
     if (PASS == 1) {
-        using D = Dual<NADJ1>;
-        const D xi[3] = {dseed<NADJ1>(me.x, 0), dseed<NADJ1>(me.y, 1), dseed<NADJ1>(me.z, 2)};
-        const D vi[3] = {dseed<NADJ1>(me.vx, 3), dseed<NADJ1>(me.vy, 4), dseed<NADJ1>(me.vz, 5)};
-        const D m0 = dseed<NADJ1>(B.m0, 6);
-        const D im0 = dinv(m0);
-        const D di[3] = {xi[0] - B.x0[0], xi[1] - B.x0[1], xi[2] - B.x0[2]};
-        const D rb2 = dot(di, di);
-        const D rbe = drsq(rb2 + P.eps2);
-        const D rbe3 = rbe * rbe * rbe;
-        const D vvi = dot(vi, vi);
         double acc[NADJ1];
-#pragma unroll
-        for (int k = 0; k < NADJ1; ++k) acc[k] = 0.0;
-        for (int j = R.j0; j < R.j1; ++j) {
-            const ARec J = rec[j];
-            const double xj[3] = {J.x, J.y, J.z}, vj[3] = {J.vx, J.vy, J.vz};
-            const double dj[3] = {J.dx, J.dy, J.dz};
-            const double rce3 = J.e * J.e * J.e;
-            const D phi = adj_pair_phi(xi, vi, di, rb2, rbe, rbe3, vvi, xj, vj, dj, J.D2, J.e,
-                                       rce3, me.m, J.m, m0, im0, ume, B.u0, P.eps2, P.lam);
-            if (j != i) {
-#pragma unroll
-                for (int k = 0; k < NADJ1; ++k) acc[k] += phi.d[k];
-            }
-        }
+        adj_pass1(rec, B, me, i, R.j0, R.j1, P.eps2, P.lam, acc);
         if (li < n) {
 #pragma unroll
             for (int k = 0; k < NADJ1; ++k) part[(size_t)(s * NADJ + k) * P.cap + li] = acc[k];
         }
     } else {
-        using D = Dual<NADJ2>;
-        const D xj[3] = {dseed<NADJ2>(me.x, 0), dseed<NADJ2>(me.y, 1), dseed<NADJ2>(me.z, 2)};
-        const D vj[3] = {dseed<NADJ2>(me.vx, 3), dseed<NADJ2>(me.vy, 4), dseed<NADJ2>(me.vz, 5)};
-        const D dj[3] = {xj[0] - B.x0[0], xj[1] - B.x0[1], xj[2] - B.x0[2]};
-        const D rc2 = dot(dj, dj);
-        const D rce = drsq(rc2 + P.eps2);
-        const D rce3 = rce * rce * rce;
-        const double im0 = 1.0 / B.m0;
         double acc[NADJ2];
-#pragma unroll
-        for (int k = 0; k < NADJ2; ++k) acc[k] = 0.0;
-        for (int j = R.j0; j < R.j1; ++j) {             // j runs over the i side here
-            const ARec I = rec[j];
-            const double xi[3] = {I.x, I.y, I.z}, vi[3] = {I.vx, I.vy, I.vz};
-            const double di[3] = {I.dx, I.dy, I.dz};
-            const double ui[3] = {I.ux, I.uy, I.uz};
-            const double rbe3 = I.e * I.e * I.e;
-            const double vvi = I.vx * I.vx + I.vy * I.vy + I.vz * I.vz;
-            const D phi = adj_pair_phi(xi, vi, di, I.D2, I.e, rbe3, vvi, xj, vj, dj, rc2, rce,
-                                       rce3, I.m, me.m, B.m0, im0, ui, B.u0, P.eps2, P.lam);
-            if (j != i) {
-#pragma unroll
-                for (int k = 0; k < NADJ2; ++k) acc[k] += phi.d[k];
-            }
-        }
+        adj_pass2(rec, B, me, i, R.j0, R.j1, P.eps2, P.lam, acc);
         if (li < n) {
 #pragma unroll
             for (int k = 0; k < NADJ2; ++k)
@@ -290,40 +355,16 @@ __global__ __launch_bounds__(WG) void k_adj_fin(const ARec* __restrict__ rec,
     const size_t cap = (size_t)P.cap;
     double t[4] = {0.0, 0.0, 0.0, 0.0};
     if (p < P.n) {
-        const ABH B = load_abh(y, mu, m, P.cap);
-        const ARec me = rec[p];
-        using D = Dual<NADJ1>;
-        const D xi[3] = {dseed<NADJ1>(me.x, 0), dseed<NADJ1>(me.y, 1), dseed<NADJ1>(me.z, 2)};
-        const D vi[3] = {dseed<NADJ1>(me.vx, 3), dseed<NADJ1>(me.vy, 4), dseed<NADJ1>(me.vz, 5)};
-        const D m0 = dseed<NADJ1>(B.m0, 6);
-        const D im0 = dinv(m0);
-        const D di[3] = {xi[0] - B.x0[0], xi[1] - B.x0[1], xi[2] - B.x0[2]};
-        const D rb2 = dot(di, di);
-        const D rbe = drsq(rb2 + P.eps2);
-        const double ume[3] = {me.ux, me.uy, me.uz};
-        const D psi = adj_star_psi(vi, di, rb2, rbe, me.m, m0, im0, ume, B.u0, P.eps, P.lam);
-        // the self pair phi_ii, both sides on the same variables: r = 0 and
-        // its tangents cancel exactly, as in the reference's own sum
-        const D rbe3 = rbe * rbe * rbe;
-        const D vvi = dot(vi, vi);
-        const D self = adj_pair_phi(xi, vi, di, rb2, rbe, rbe3, vvi, xi, vi, di, rb2, rbe, rbe3,
-                                    me.m, me.m, m0, im0, ume, B.u0, P.eps2, P.lam);
+        const ABH B = load_abh(y, mu, m, P.cap, 0);
         double g[NADJ1];
-#pragma unroll
-        for (int k = 0; k < NADJ1; ++k) g[k] = psi.d[k] + self.d[k];
+        adj_star_grad(rec[p], B, P.eps, P.eps2, P.lam, g);
         for (int s = 0; s < S; ++s) {
 #pragma unroll
             for (int k = 0; k < NADJ1; ++k) g[k] += part[(size_t)(s * NADJ + k) * cap + p];
 #pragma unroll
             for (int k = 0; k < NADJ2; ++k) g[k] += part[(size_t)(s * NADJ + NADJ1 + k) * cap + p];
         }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            out[(size_t)k * cap + p] = g[k];
-            out[(size_t)(3 + k) * cap + p] = mu[(size_t)k * cap + p] + g[3 + k];
-            t[k] = g[k];
-        }
-        t[3] = g[6];
+        adj_store(g, mu, out, cap, p, t);
     }
     block_sum<4>(t, blk + (size_t)blockIdx.x * 4);
 }

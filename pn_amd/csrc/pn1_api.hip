@@ -24,6 +24,7 @@
 
 #include "pn1.h"
 #include "pn1_ctx.hpp"
+#include "pn1_ens.hpp"
 #include "pn1_ens_kernels.hpp"
 #include "pn1_kernels.hpp"
 

@@ -86,6 +86,9 @@ def load():
                                     ctypes.POINTER(_I)]),
         "pn1_ens_profile_enable": (_I, [P, _I]),
         "pn1_ens_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
+        "pn1_ens_force_vjp": (_I, [P, _D, _D, _D, _D]),
+        "pn1_ens_adjoint_forward": (_I, [P, _I, _I]),
+        "pn1_ens_adjoint_backward": (_I, [P, _D, _D, _D, _D, _D]),
         "pn1_check_input": (_I, [_D, _D, _D, _I, ctypes.c_double, ctypes.c_char_p, _I]),
         "pn1_ic_uniform_sphere": (_I, [_I, ctypes.c_ulonglong, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, _D, _D, _D]),
@@ -339,6 +342,35 @@ class Ensemble:
         self._chk(self._L.pn1_ens_profile_read(self._h, ctypes.byref(k), ctypes.byref(ms)),
                   "pn1_ens_profile_read")
         return k.value, ms.value
+
+    # ---- adjoint
+    def _cat3(self, parts):
+        return _f64(np.concatenate([np.asarray(p, dtype=np.float64).reshape(-1, 3)
+                                    for p in parts]), (self.n, 3))
+
+    def force_vjp(self, us):
+        """Per cluster (gx, gv) lists and the m0 derivatives, for the
+        per-cluster cotangents `us`."""
+        u = self._cat3(us)
+        gx = np.empty((self.n, 3))
+        gv = np.empty((self.n, 3))
+        gm0 = np.empty(len(self.offsets) - 1)
+        self._chk(self._L.pn1_ens_force_vjp(self._h, ptr(u), ptr(gx), ptr(gv), ptr(gm0)),
+                  "pn1_ens_force_vjp")
+        return self.split(gx), self.split(gv), gm0
+
+    def adjoint_forward(self, steps, interval=16):
+        self._chk(self._L.pn1_ens_adjoint_forward(self._h, int(steps), int(interval)),
+                  "pn1_ens_adjoint_forward")
+
+    def adjoint_backward(self, gxs, gvs):
+        gx, gv = self._cat3(gxs), self._cat3(gvs)
+        gx0 = np.empty((self.n, 3))
+        gv0 = np.empty((self.n, 3))
+        gm0 = np.empty(len(self.offsets) - 1)
+        self._chk(self._L.pn1_ens_adjoint_backward(self._h, ptr(gx), ptr(gv), ptr(gx0), ptr(gv0),
+                                                   ptr(gm0)), "pn1_ens_adjoint_backward")
+        return self.split(gx0), self.split(gv0), gm0
 
 
 # ---------------------------------------------------------------- legacy shapes

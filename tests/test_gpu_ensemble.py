@@ -20,6 +20,7 @@ import pytest
 import oracle_bind as ob
 from conftest import energy_atol, rel_err
 from pn_amd import lib as pn
+from pn_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -210,3 +211,59 @@ def test_upload_rejects_bad_layouts(ens, clusters):
     # the previous good state is still usable after a rejected upload
     ens.upload(clusters[:2])
     assert len(ens.force()) == 2
+
+
+# ---------------------------------------------------------------- adjoint
+def test_force_vjp_bitwise_equals_single_cluster(ens, ctx, clusters):
+    """With one chunk per cluster (and an unsplit single-context j range)
+    the ensemble VJP runs each cluster's sums in the same order."""
+    rng = np.random.default_rng(41)
+    us = [rng.normal(size=(len(m), 3)) for _, _, m in clusters]
+    ens.set_tuning(4096, 0, -1)
+    ens.set_params(EPS, C, DT, pn.FP64)
+    ens.upload(clusters)
+    gxs, gvs, gm0 = ens.force_vjp(us)
+    for k, cl in enumerate(clusters):
+        ctx.set_tuning(1, 0, -1)
+        ctx.set_params(EPS, C, DT, pn.FP64)
+        ctx.upload(*cl)
+        gx, gv, g0 = ctx.force_vjp(us[k])
+        assert np.array_equal(gxs[k], gx) and np.array_equal(gvs[k], gv), k
+        assert gm0[k] == g0, (k, gm0[k], g0)
+
+
+def ens_loss_grads(ens, r_cut):
+    xs, vs = ens.download()
+    out = [workloads.radial_loss(x, v, r_cut) for x, v in zip(xs, vs)]
+    return [o[1] for o in out], [o[2] for o in out]
+
+
+@pytest.mark.parametrize("chunk", [4096, 0])
+def test_adjoint_equals_single_cluster(ens, ctx, clusters, chunk):
+    """Trajectory gradient of every cluster (its own radial loss) through the
+    ensemble adjoint: bitwise the single-context result with matching
+    shapes, within 1e-12 with the automatic chunk."""
+    sub = clusters[:9]            # up to 257 stars: keeps the test short
+    steps, interval = 6, 4
+    ens.set_tuning(chunk, 0, -1)
+    ens.set_params(EPS, C, DT, pn.MIXED)      # the adjoint runs FP64 regardless
+    ens.upload(sub)
+    ens.adjoint_forward(steps, interval)
+    lx, lv = ens_loss_grads(ens, 0.8)
+    gx0s, gv0s, gm0 = ens.adjoint_backward(lx, lv)
+    for k, cl in enumerate(sub):
+        ctx.set_tuning(1, 0, -1)
+        ctx.set_params(EPS, C, DT, pn.FP64)
+        ctx.upload(*cl)
+        ctx.adjoint_forward(steps, interval)
+        xt, vt = ctx.download()
+        _, gx, gv = workloads.radial_loss(xt, vt, 0.8)
+        gx0, gv0, g0 = ctx.adjoint_backward(gx, gv)
+        if chunk:
+            assert np.array_equal(gx0s[k], gx0) and np.array_equal(gv0s[k], gv0), k
+            assert gm0[k] == g0, k
+        else:
+            scale = max(np.abs(gx0).max(), np.abs(gv0).max(), abs(g0), 1e-300)
+            assert np.abs(gx0s[k] - gx0).max() <= 1e-12 * scale, k
+            assert np.abs(gv0s[k] - gv0).max() <= 1e-12 * scale, k
+            assert abs(gm0[k] - g0) <= 1e-12 * scale, k
