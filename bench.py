@@ -281,6 +281,33 @@ def run_adjoint(steps, interval, vjp_reps=5):
     }
 
 
+def run_drift(x, v, m, steps, every=100):
+    """Relative energy drift (E_t - E_0) / |E_0| of both forms over `steps`
+    steps of the headline workload, sampled every `every` steps, with the
+    mixed form's per-particle acceleration error at the start."""
+    out = {"workload": "Plummer N=131072 + BH m=0.05, c=50, eps=1e-4, dt=2^-12",
+           "steps": steps, "every": every}
+    n = len(m)
+    with pn.Context(capacity=n, device=0) as ctx:
+        for mode, name in ((pn.FP64, "fp64"), (pn.MIXED, "mixed")):
+            ctx.set_params(EPS, C_LIGHT, DT, mode)
+            ctx.upload(x, v, m)
+            e0 = ctx.energy()
+            series = []
+            t0 = time.perf_counter()
+            for k in range(0, steps, every):
+                ctx.step(min(every, steps - k))
+                e = ctx.energy()
+                series.append(float((e[4] - e0[4]) / abs(e0[4])))
+                print(f"# {name} step {k + min(every, steps - k)}: drift {series[-1]:+.6e}",
+                      file=sys.stderr, flush=True)
+            out[name] = {"drift": series, "final": series[-1], "E0": float(e0[4]),
+                         "wall_s": time.perf_counter() - t0}
+    out["mixed_over_fp64_final"] = out["mixed"]["final"] / out["fp64"]["final"]
+    out["mixed_minus_fp64_final"] = out["mixed"]["final"] - out["fp64"]["final"]
+    return out
+
+
 def roofline_flops(inter, avg_ms, peak, traffic=None):
     achieved = FLOPS_PER_INTERACTION * inter / (avg_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak / 1e12,
@@ -304,6 +331,9 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel of the ensemble leg directly instead of "
                          "replaying captured step graphs (for profilers)")
+    ap.add_argument("--drift-steps", type=int, default=0,
+                    help="only measure the energy drift of both forms over this many steps "
+                         "of the headline workload, print it as JSON and exit")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel over MIB MiB, 3 times (for "
                          "calibrating profiler byte counters), and exit")
@@ -320,6 +350,9 @@ def main():
 
     x, v, m = pn.ic_plummer(N_STARS, SEED, a=1.0, m_bh=M_BH)
     n = len(m)
+    if args.drift_steps:
+        print(json.dumps(run_drift(x, v, m, args.drift_steps)), flush=True)
+        return
     with pn.Context(capacity=n, device=0) as ctx:
         ctx.set_tuning(args.jsplit, args.tile, args.variant)
         # accuracy of the mixed form at this state (one evaluation each)
