@@ -397,6 +397,9 @@ def main():
         "pair_kernel_avg_ms": {"mixed": mixed["pair_avg_ms"], "fp64": fp64["pair_avg_ms"],
                                "launches": mixed["pair_launches"]},
         "flops_per_interaction": FLOPS_PER_INTERACTION,
+        # the literature's Newtonian-only convention (20 flops per interaction)
+        "tflops_newtonian_convention": {"mixed": 20.0 * mixed["interactions_per_s"] / 1e12,
+                                        "fp64": 20.0 * fp64["interactions_per_s"] / 1e12},
     }
     if args.ens_steps > 0:
         out["ensemble"] = run_ensemble(args.ens_steps, args.warmup, chunk=args.ens_chunk,
