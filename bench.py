@@ -51,16 +51,17 @@ FLOPS_PER_INTERACTION = 90
 PEAK_FP32 = 157.3e12   # MI355X vector FP32
 PEAK_FP64 = 78.6e12    # MI355X vector FP64
 CPU_SAMPLE_STARS = 16384
-ENS_TRAFFIC = "ensemble_pair_traffic_c5.json"   # measured at chunk 512
+ENS_TRAFFIC = "ensemble_pair_traffic_c5.json"
 
 
 def rel_err(a, ref):
     return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
 
 
-def run_form(ctx, x, v, m, mode, steps, warmup):
+def run_form(ctx, x, v, m, mode, steps, warmup, autotune=True):
     ctx.set_params(EPS, C_LIGHT, DT, mode)
     ctx.upload(x, v, m)
+    tuned = ctx.autotune(2) if autotune else None     # before the timed region
     e0 = ctx.energy()
     ctx.profile_enable(True)
     if warmup:
@@ -82,6 +83,8 @@ def run_form(ctx, x, v, m, mode, steps, warmup):
         pair_launches=launches,
         pair_avg_ms=pair_ms / max(launches, 1),
         energy_drift=float((e1[4] - e0[4]) / abs(e0[4])),
+        shape=ctx.get_tuning(),
+        autotune_best_ms=tuned[2] if tuned else None,
     )
 
 
@@ -92,14 +95,24 @@ def roofline(n, avg_ms, peak, traffic=None):
             "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4), "traffic": traffic}
 
 
-def load_traffic(kind, name="pair_kernel_traffic_n131073.json"):
-    """HBM bytes per pair launch from a committed rocprofv3 --pmc summary."""
+def load_traffic(kind, name="pair_kernel_traffic_n131073.json", shape=None):
+    """HBM bytes per pair launch from a committed rocprofv3 --pmc summary,
+    if it was measured at this launch shape (the partial sums written scale
+    with the j split / chunk length); else None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
-            return json.load(f).get(kind)
+            d = json.load(f)
     except (OSError, ValueError):
         return None
+    if shape is not None and "chunk" in shape and "by_chunk" in d:
+        row = d["by_chunk"].get(kind, {}).get(str(shape["chunk"]))
+        return row["total"] if row else None
+    want = d.get("shape", {}).get(kind)
+    if shape is not None and want is not None:
+        if any(shape.get(k) != v for k, v in want.items()):
+            return None
+    return d.get(kind)
 
 
 def cpu_baseline(x, v, m):
@@ -148,7 +161,8 @@ def time_steps(obj, steps, warmup, eager=False):
     return wall
 
 
-def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_steps=0):
+def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_steps=0,
+                 autotune=True):
     """The batched workload: 256 seeded Plummer clusters (star counts
     log-uniform in [256, 8192], BH masses uniform in [0.005, 0.05], c = 100,
     eps = 1e-3, dt = 2^-10), all advanced by one pn1_ens_step per step (one
@@ -162,13 +176,18 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
     res = {}
     with pn.Ensemble(capacity=ntot, max_clusters=len(cl)) as ens:
         ens.set_tuning(chunk, tile, variant)
+        shapes = {}
         for mode, name in ((pn.FP64, "fp64"), (pn.MIXED, "mixed")):
+            ens.set_tuning(chunk, tile, variant)
             ens.set_params(W["eps"], W["c"], W["dt"], mode)
             ens.upload(cl)
+            if autotune:
+                ens.autotune(2)                     # before the timed region
+            shapes[name] = ens.get_tuning()
             e0 = ens.energy()
             wall = time_steps(ens, steps, warmup, eager)
             e1 = ens.energy()
-            ens.upload(cl)
+            ens.upload(cl)                          # keeps the tuned shape
             ens.profile_enable(True)
             ens.step(1)
             k, ms = ens.profile_read()
@@ -183,7 +202,6 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
         a64 = np.concatenate(ens.force())
         ens.set_params(W["eps"], W["c"], W["dt"], pn.MIXED)
         a32 = np.concatenate(ens.force())
-        chunk_used, tile_used, var_used = ens.get_tuning()
         # the BH-mass gradient of every cluster's radial loss (FP64 adjoint)
         adj = None
         if adj_steps > 0:
@@ -199,7 +217,11 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
                    "loss": "per cluster: mean squared radial velocity within r < 0.2 of its BH",
                    "forward_s": t1 - t0, "backward_s": t2 - t1,
                    "grad_m0_median_abs": float(np.median(np.abs(gm0))),
-                   "grad_m0_max_abs": float(np.abs(gm0).max())}
+                   "grad_m0_max_abs": float(np.abs(gm0).max()),
+                   # clusters where a star passes close to the BH, whose 1PN term
+                   # has an unsoftened 1/|d|^2 (Limit-sticky9.c:74): the step
+                   # map's derivative there is huge, and so is the gradient
+                   "clusters_grad_m0_above_1e3": int((np.abs(gm0) > 1e3).sum())}
     err = rel_err(a32, a64)
     sep_wall = 0.0
     with pn.Context(capacity=max(len(m) for _, _, m in cl)) as ctx:
@@ -219,17 +241,21 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
         "separate_interactions_per_s": sep,
         "ensemble_over_separate": mixed["interactions_per_s"] / sep,
         "roofline": roofline_flops(inter, mixed["pair_avg_ms"], PEAK_FP32,
-                                   load_traffic("mixed", ENS_TRAFFIC) if chunk_used == 512 else None),
+                                   load_traffic("mixed", ENS_TRAFFIC,
+                                                {"chunk": shapes["mixed"][0],
+                                                 "variant": shapes["mixed"][2]})),
         "fp64": {"interactions_per_s": res["fp64"]["interactions_per_s"],
                  "roofline": roofline_flops(inter, res["fp64"]["pair_avg_ms"], PEAK_FP64,
-                                            load_traffic("fp64", ENS_TRAFFIC)
-                                            if chunk_used == 512 else None),
+                                            load_traffic("fp64", ENS_TRAFFIC,
+                                                         {"chunk": shapes["fp64"][0],
+                                                          "variant": shapes["fp64"][2]})),
                  "energy_drift_max": res["fp64"]["energy_drift_max"]},
         "mixed_accuracy_vs_fp64": {"rel_err_max": float(err.max()),
                                    "rel_err_p999": float(np.quantile(err, 0.999)),
                                    "energy_drift_max": mixed["energy_drift_max"],
                                    "energy_drift_median": mixed["energy_drift_median"]},
-        "tuning": {"chunk": chunk_used, "tile": tile_used, "variant": var_used},
+        "tuning": {"mixed": list(shapes["mixed"]), "fp64": list(shapes["fp64"]),
+                   "chosen_by": "pn1_ens_autotune" if autotune else "automatic chunk"},
         "adjoint_bh_mass": adj,
     }
 
@@ -326,6 +352,11 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=128,
                     help="steps of the batched-ensemble workload (0 skips it)")
     ap.add_argument("--ens-chunk", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the headline pair kernels' launch shapes with pn1_autotune "
+                         "(default: the automatic plan, whose HBM traffic is profiled)")
+    ap.add_argument("--no-ens-autotune", action="store_true",
+                    help="use the ensemble's automatic chunk instead of pn1_ens_autotune")
     ap.add_argument("--adj-steps", type=int, default=256,
                     help="steps of the differentiation workload (0 skips it)")
     ap.add_argument("--eager", action="store_true",
@@ -363,9 +394,10 @@ def main():
         ctx.upload(x, v, m)
         a32 = ctx.force()
         err = rel_err(a32, a64)
-        fp64 = run_form(ctx, x, v, m, pn.FP64, args.steps, args.warmup)
-        mixed = run_form(ctx, x, v, m, pn.MIXED, args.steps, args.warmup)
-        jsplit, tile, variant = ctx.get_tuning()
+        tune = args.jsplit == 0 and args.variant == -1 and args.autotune
+        fp64 = run_form(ctx, x, v, m, pn.FP64, args.steps, args.warmup, tune)
+        mixed = run_form(ctx, x, v, m, pn.MIXED, args.steps, args.warmup, tune)
+        jsplit, tile, variant = mixed["shape"]
 
     out = {
         "metric": METRIC,
@@ -383,11 +415,16 @@ def main():
         "config": {"workload": "Plummer N=131072 + BH m=0.05, c=50, eps=1e-4, dt=2^-12, "
                                "3 force evaluations per step",
                    "n": n, "interactions_per_step": 3 * n * n, "jsplit": jsplit, "tile": tile,
-                   "variant": variant},
-        "roofline": roofline(n, mixed["pair_avg_ms"], PEAK_FP32, load_traffic("mixed")),
+                   "variant": variant, "fp64_shape": list(fp64["shape"]),
+                   "shape_chosen_by": "pn1_autotune" if tune else "fixed / automatic plan"},
+        "roofline": roofline(n, mixed["pair_avg_ms"], PEAK_FP32,
+                             load_traffic("mixed", shape={"jsplit": jsplit, "variant": variant,
+                                                          "tile": tile})),
         "fp64": {"interactions_per_s": fp64["interactions_per_s"],
                  "ms_per_step": fp64["ms_per_step"],
-                 "roofline": roofline(n, fp64["pair_avg_ms"], PEAK_FP64, load_traffic("fp64")),
+                 "roofline": roofline(n, fp64["pair_avg_ms"], PEAK_FP64,
+                                      load_traffic("fp64", shape={"jsplit": fp64["shape"][0],
+                                                                  "variant": fp64["shape"][2]})),
                  "energy_drift": fp64["energy_drift"]},
         "mixed_over_fp64": mixed["interactions_per_s"] / fp64["interactions_per_s"],
         "mixed_accuracy_vs_fp64": {"star_rel_err_max": float(err[1:].max()),
@@ -404,7 +441,8 @@ def main():
     if args.ens_steps > 0:
         out["ensemble"] = run_ensemble(args.ens_steps, args.warmup, chunk=args.ens_chunk,
                                        tile=args.tile, variant=args.variant, eager=args.eager,
-                                       adj_steps=args.ens_steps if args.adj_steps > 0 else 0)
+                                       adj_steps=args.ens_steps if args.adj_steps > 0 else 0,
+                                       autotune=not args.no_ens_autotune and args.ens_chunk == 0)
     if args.adj_steps > 0:
         out["adjoint"] = run_adjoint(args.adj_steps, min(16, args.adj_steps))
     if not args.no_cpu:

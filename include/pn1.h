@@ -110,6 +110,13 @@ const char *pn1_last_error(const pn1_ctx *ctx);
  * rsqrt.  pn1_get_tuning reports the values in use.  Used by the tuner. */
 int  pn1_set_tuning(pn1_ctx *ctx, int jsplit, int tile, int variant);
 int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile, int *variant);
+/* Autotuner: times the pair kernel of `reps` force evaluations of the
+ * resident state for each j split (automatic, 8 ... 256) and j loop
+ * variant, keeps the fastest (as pn1_set_tuning would) and reports it
+ * (any output may be NULL).  The mixed form's compensation interval is
+ * not changed: it trades accuracy.  No counterpart in the reference. */
+int  pn1_autotune(pn1_ctx *ctx, int reps, int *jsplit, int *variant,
+                  double *best_ms);
 
 /* Timing of the pair kernel: while enabled, pn1_step/pn1_force launch
  * without a graph and bracket every force pair-kernel launch with HIP
@@ -190,6 +197,11 @@ const char *pn1_ens_last_error(const pn1_ens *ens);
 int  pn1_ens_set_tuning(pn1_ens *ens, int chunk, int tile, int variant);
 int  pn1_ens_get_tuning(const pn1_ens *ens, int *chunk, int *tile,
                         int *variant);
+/* Autotuner of the ensemble, as pn1_autotune, over chunk lengths
+ * 256 ... 1024 and the j loop variant; applies at once (the work tables
+ * are rebuilt) and invalidates a pending adjoint. */
+int  pn1_ens_autotune(pn1_ens *ens, int reps, int *chunk, int *variant,
+                      double *best_ms);
 /* Adjoint, per cluster, as pn1_force_vjp / pn1_adjoint_forward /
  * pn1_adjoint_backward (FP64 whatever the mode); gm0 (may be NULL) gets
  * one value per cluster: the derivative with respect to that cluster's BH

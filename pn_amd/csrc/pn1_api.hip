@@ -522,6 +522,61 @@ int pn1_profile_read(pn1_ctx* c, int* launches, double* total_ms)
     return PN1_OK;
 }
 
+// Autotuner: times the pair kernel of `reps` force evaluations on the
+// resident state per candidate shape (j split x kernel variant; the mixed
+// form's compensation interval is left alone: it trades accuracy), after
+// one warm-up evaluation, and keeps the fastest.
+int pn1_autotune(pn1_ctx* c, int reps, int* jsplit, int* variant, double* best_ms)
+{
+    int rc;
+    if (!ready(c, &rc)) return rc;
+    if (reps < 1) return seterr(c, PN1_EINVAL, "reps must be >= 1");
+    const int nst = c->n - 1;
+    const bool was_prof = c->prof;
+    const int keep_tile = c->tile_req, keep_S = c->jsplit_req, keep_var = c->variant_req;
+    std::vector<int> splits = {0};                  // 0 = the automatic plan
+    for (int S = 8; S <= 256; S *= 2)
+        if (S <= std::max(1, nst / 64)) splits.push_back(S);
+    const Phys P = make_phys(c);
+    double best = 1e300;
+    int bS = 0, bV = -1;
+    for (int S : splits) {
+        for (int var = 0; var < 2; ++var) {
+            c->jsplit_req = S;
+            c->variant_req = var;
+            c->tile_req = keep_tile;
+            plan(c);
+            if ((rc = ensure_part(c))) return rc;
+            if ((rc = pn1_profile_enable(c, 1))) return rc;
+            if ((rc = enqueue_force(c, 0, P))) return rc;          // warm-up
+            if ((rc = pn1_profile_enable(c, 1))) return rc;
+            for (int r = 0; r < reps; ++r)
+                if ((rc = enqueue_force(c, 0, P))) return rc;
+            int k = 0;
+            double ms = 0.0;
+            if ((rc = pn1_profile_read(c, &k, &ms))) return rc;
+            const double avg = k ? ms / k : 0.0;
+            if (avg < best) {
+                best = avg;
+                bS = c->S;
+                bV = var;
+            }
+        }
+    }
+    (void)keep_S;
+    (void)keep_var;
+    if ((rc = pn1_profile_enable(c, was_prof ? 1 : 0))) return rc;
+    c->jsplit_req = bS;
+    c->variant_req = bV;
+    drop_graph(c);
+    plan(c);
+    if ((rc = ensure_part(c))) return rc;
+    if (jsplit) *jsplit = c->S;
+    if (variant) *variant = kernel_variant(c);
+    if (best_ms) *best_ms = best;
+    return PN1_OK;
+}
+
 // Copies `bytes` (rounded down to 16) between two device buffers with
 // k_copy on the context's stream, `reps` times; for counter calibration.
 int pn1_util_copy(pn1_ctx* c, long long bytes, int reps)

@@ -53,6 +53,7 @@ struct pn1_ens {
     double prof_ms = 0.0;
     pn1_ens_adj* adj = nullptr;   // allocated by the first adjoint call
     unsigned gen = 0;             // bumped when the state or eps, c, dt change
+    std::vector<int> off_h;       // host copy of the cluster offsets
     char err[512] = {0};
 };
 

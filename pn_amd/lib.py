@@ -68,6 +68,8 @@ def load():
         "pn1_profile_enable": (_I, [P, _I]),
         "pn1_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
         "pn1_util_copy": (_I, [P, ctypes.c_longlong, _I]),
+        "pn1_autotune": (_I, [P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _D]),
+        "pn1_ens_autotune": (_I, [P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _D]),
         "pn1_force_vjp": (_I, [P, _D, _D, _D, _D]),
         "pn1_adjoint_forward": (_I, [P, _I, _I]),
         "pn1_adjoint_backward": (_I, [P, _D, _D, _D, _D, _D]),
@@ -167,6 +169,13 @@ class Context:
         self._chk(self._L.pn1_get_tuning(self._h, ctypes.byref(s), ctypes.byref(t),
                                          ctypes.byref(v)), "pn1_get_tuning")
         return s.value, t.value, v.value
+
+    def autotune(self, reps=3):
+        """(jsplit, variant, pair ms) of the fastest shape, now in use."""
+        s, v, ms = _I(), _I(), ctypes.c_double()
+        self._chk(self._L.pn1_autotune(self._h, int(reps), ctypes.byref(s), ctypes.byref(v),
+                                       ctypes.byref(ms)), "pn1_autotune")
+        return s.value, v.value, ms.value
 
     def upload(self, x, v, m):
         m = _f64(m)
@@ -297,6 +306,13 @@ class Ensemble:
         self._chk(self._L.pn1_ens_get_tuning(self._h, ctypes.byref(s), ctypes.byref(t),
                                              ctypes.byref(v)), "pn1_ens_get_tuning")
         return s.value, t.value, v.value
+
+    def autotune(self, reps=3):
+        """(chunk, variant, pair ms) of the fastest shape, now in use."""
+        s, v, ms = _I(), _I(), ctypes.c_double()
+        self._chk(self._L.pn1_ens_autotune(self._h, int(reps), ctypes.byref(s), ctypes.byref(v),
+                                           ctypes.byref(ms)), "pn1_ens_autotune")
+        return s.value, v.value, ms.value
 
     def upload(self, clusters):
         sizes = [len(m) for _, _, m in clusters]

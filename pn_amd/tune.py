@@ -78,14 +78,14 @@ def sweep(ctx, x, v, m, eps, c, modes, reps=3, jsplits=(8, 16, 32, 64, 128),
 
 
 def ens_sweep(ens, clusters, eps, c, mode, reps=3, chunks=(64, 128, 256, 512, 1024, 2048),
-              tiles=(16, 32), variants=(0, 1), out=sys.stdout):
+              tiles=(16, 32), variants=(0, 1), out=sys.stdout, shapes=None):
     """Ensemble form: times the persistent pair kernel per work-item chunk
     length (and, mixed, compensation interval and variant)."""
     inter = sum(float(len(m)) ** 2 for _, _, m in clusters)
     rows = []
     name = "fp64" if mode == pn.FP64 else "mixed"
     tl = (0,) if mode == pn.FP64 else tiles
-    for shape in [(0, 0, -1)] + list(itertools.product(chunks, tl, variants)):
+    for shape in shapes or [(0, 0, -1)] + list(itertools.product(chunks, tl, variants)):
         ens.set_tuning(*shape)
         ens.set_params(eps, c, 2.0**-10, mode)
         ens.upload(clusters)
@@ -121,6 +121,9 @@ def main(argv=None):
     ap.add_argument("--mode", choices=("fp64", "mixed", "both"), default="both")
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--shape", default=None, metavar="A,TILE,VARIANT",
+                    help="time only this shape (A = j split, or chunk length with "
+                         "--workload ensemble); for profiling one launch shape")
     args = ap.parse_args(argv)
     if args.workload == "ensemble":
         from . import workloads
@@ -132,9 +135,11 @@ def main(argv=None):
               f"eps={W['eps']} c={W['c']}, {args.reps} timed force evaluations per shape",
               flush=True)
         result = {}
+        one = [tuple(int(t) for t in args.shape.split(","))] if args.shape else None
         with pn.Ensemble(ntot, len(clusters)) as ens:
             for mode in modes:
-                rows, best = ens_sweep(ens, clusters, W["eps"], W["c"], mode, reps=args.reps)
+                rows, best = ens_sweep(ens, clusters, W["eps"], W["c"], mode, reps=args.reps,
+                                       shapes=one)
                 result["fp64" if mode == pn.FP64 else "mixed"] = dict(rows=rows, best=best)
         if args.json:
             with open(args.json, "w") as f:

@@ -215,3 +215,35 @@ def test_legacy_energy_text(tmp_path):
         digits = len(b.split("e")[0].split(".")[1])             # %22.15e fields vs %e fields
         tol = 1e-12 if digits == 15 else 2e-6
         assert abs(float(a) - float(b)) <= tol * abs(float(b)), (a, b)
+
+
+def test_autotune_keeps_parity():
+    """pn1_autotune times the candidate launch shapes on the resident state,
+    keeps the fastest, and the forces stay within the FP64 bound."""
+    import oracle_bind as ob
+    x, v, m = pn.ic_plummer(4096, 31, a=1.0, m_bh=0.05)
+    with pn.Context(len(m)) as ctx:
+        ctx.set_params(1e-3, 50.0, 2.0**-10, pn.FP64)
+        ctx.upload(x, v, m)
+        S, var, ms = ctx.autotune(2)
+        assert ms > 0.0
+        assert ctx.get_tuning()[0] == S and ctx.get_tuning()[2] == var
+        a = ctx.force()
+    ref = ob.orc_force(x, v, m, 1e-3, 50.0)
+    err = np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert err.max() <= 1e-12, err.max()
+
+
+def test_ens_autotune_keeps_results():
+    x, v, m = pn.ic_plummer(1000, 32, a=1.0, m_bh=0.05)
+    cl = [pn.ic_plummer(ns, 33 + k, a=1.0, m_bh=0.02) for k, ns in enumerate((300, 2000, 700))]
+    n = sum(len(mm) for _, _, mm in cl)
+    with pn.Ensemble(n, len(cl)) as ens:
+        ens.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+        ens.upload(cl)
+        a0 = ens.force()
+        L, var, ms = ens.autotune(2)
+        assert ms > 0.0 and ens.get_tuning()[0] == L and 256 <= L <= 1024
+        a1 = ens.force()
+    for p, q in zip(a0, a1):
+        assert np.abs(p - q).max() <= 1e-13 * np.abs(p).max()
