@@ -50,7 +50,7 @@ N_STARS, SEED, M_BH, C_LIGHT, EPS, DT = 131072, 3, 0.05, 50.0, 1e-4, 2.0**-12
 FLOPS_PER_INTERACTION = 90
 PEAK_FP32 = 157.3e12   # MI355X vector FP32
 PEAK_FP64 = 78.6e12    # MI355X vector FP64
-CPU_SAMPLE_STARS = 16384
+CPU_SAMPLE_STARS = 24576
 ENS_TRAFFIC = "ensemble_pair_traffic_c5.json"
 
 
