@@ -540,32 +540,42 @@ int pn1_autotune(pn1_ctx* c, int reps, int* jsplit, int* variant, double* best_m
     const Phys P = make_phys(c);
     double best = 1e300;
     int bS = 0, bV = -1;
+    auto time_shape = [&](int S, int var, double* avg) {
+        c->jsplit_req = S;
+        c->variant_req = var;
+        c->tile_req = keep_tile;
+        plan(c);
+        int r = ensure_part(c);
+        if (!r) r = pn1_profile_enable(c, 1);
+        if (!r) r = enqueue_force(c, 0, P);                      // warm-up
+        if (!r) r = pn1_profile_enable(c, 1);
+        for (int k = 0; k < reps && !r; ++k) r = enqueue_force(c, 0, P);
+        int k = 0;
+        double ms = 0.0;
+        if (!r) r = pn1_profile_read(c, &k, &ms);
+        *avg = k ? ms / k : 0.0;
+        return r;
+    };
     for (int S : splits) {
-        for (int var = 0; var < 2; ++var) {
-            c->jsplit_req = S;
-            c->variant_req = var;
-            c->tile_req = keep_tile;
-            plan(c);
-            if ((rc = ensure_part(c))) return rc;
-            if ((rc = pn1_profile_enable(c, 1))) return rc;
-            if ((rc = enqueue_force(c, 0, P))) return rc;          // warm-up
-            if ((rc = pn1_profile_enable(c, 1))) return rc;
-            for (int r = 0; r < reps; ++r)
-                if ((rc = enqueue_force(c, 0, P))) return rc;
-            int k = 0;
-            double ms = 0.0;
-            if ((rc = pn1_profile_read(c, &k, &ms))) return rc;
-            const double avg = k ? ms / k : 0.0;
-            if (avg < best) {
+        for (int var = 0; var < 2 && !rc; ++var) {
+            double avg = 0.0;
+            rc = time_shape(S, var, &avg);
+            if (!rc && avg < best) {
                 best = avg;
                 bS = c->S;
                 bV = var;
             }
         }
+        if (rc) break;
     }
-    (void)keep_S;
-    (void)keep_var;
-    if ((rc = pn1_profile_enable(c, was_prof ? 1 : 0))) return rc;
+    (void)pn1_profile_enable(c, was_prof ? 1 : 0);
+    if (rc) {                              // leave the shape the caller had
+        c->jsplit_req = keep_S;
+        c->variant_req = keep_var;
+        plan(c);
+        (void)ensure_part(c);
+        return rc;
+    }
     c->jsplit_req = bS;
     c->variant_req = bV;
     drop_graph(c);

@@ -5,7 +5,7 @@
  *   sticky9 [--in FILE | --ic sphere|plummer --nstars N --seed S]
  *           [--n N] [--m0 M] [--eps E] [--dt DT] [--c C] [--endtime T]
  *           [--deouttime T] [--steps K] [--out FILE] [--mode fp64|mixed]
- *           [--legacy] [--checkpoint FILE] [--resume FILE]
+ *           [--legacy] [--autotune] [--checkpoint FILE] [--resume FILE]
  *
  * Defaults are the reference's: --in Init10k-sphere.dat, n = 10001,
  * m0 = 100/n and star masses 1/(n-1) (Limit-sticky9.c:813-816), eps =
@@ -17,6 +17,8 @@
  * --legacy runs the reference's own loop on host arrays, calling the
  * drop-in force() three times per step and energy() at each snapshot;
  * without it the state stays on the device (pn1_step between snapshots).
+ * --autotune (resident mode) times the pair kernel's launch shapes on the
+ * initial state first and keeps the fastest (pn1_autotune).
  * --steps K stops after K steps instead of at endtime.  --checkpoint
  * writes an exact binary state at the end; --resume starts from one.
  */
@@ -29,7 +31,7 @@
 
 typedef struct {
     const char *in, *out, *ic, *ckpt, *resume;
-    int n, nstars, mode, legacy;
+    int n, nstars, mode, legacy, autotune;
     long long steps;
     unsigned long long seed;
     double m0, eps, dt, c, endtime, deouttime;
@@ -41,7 +43,7 @@ static void usage(void)
             "usage: sticky9 [--in FILE | --ic sphere|plummer --nstars N --seed S] [--n N]\n"
             "               [--m0 M] [--eps E] [--dt DT] [--c C] [--endtime T] [--deouttime T]\n"
             "               [--steps K] [--out FILE] [--mode fp64|mixed] [--legacy]\n"
-            "               [--checkpoint FILE] [--resume FILE]\n");
+            "               [--autotune] [--checkpoint FILE] [--resume FILE]\n");
     exit(2);
 }
 
@@ -86,6 +88,7 @@ static void parse(int argc, char **argv, opts_t *o)
         else if (!strcmp(a, "--deouttime")) o->deouttime = need_num(argc, argv, &i);
         else if (!strcmp(a, "--steps")) o->steps = (long long)need_num(argc, argv, &i);
         else if (!strcmp(a, "--legacy")) o->legacy = 1;
+        else if (!strcmp(a, "--autotune")) o->autotune = 1;
         else if (!strcmp(a, "--mode") && i + 1 < argc) {
             const char *m = argv[++i];
             if (!strcmp(m, "fp64")) o->mode = PN1_FP64;
@@ -255,6 +258,14 @@ int main(int argc, char **argv)
         if (rc != PN1_OK) die_ctx("pn1_create", ctx, rc);
         if ((rc = pn1_set_params(ctx, o.eps, o.c, o.dt, o.mode)) != PN1_OK) die_ctx("pn1_set_params", ctx, rc);
         if ((rc = pn1_upload(ctx, x, v, m, n)) != PN1_OK) die_ctx("pn1_upload", ctx, rc);
+        if (o.autotune) {
+            int S = 0, var = 0;
+            double ms = 0.0;
+            if ((rc = pn1_autotune(ctx, 2, &S, &var, &ms)) != PN1_OK)
+                die_ctx("pn1_autotune", ctx, rc);
+            fprintf(stderr, "sticky9: autotuned j split %d, variant %d (%.3f ms per pair launch)\n",
+                    S, var, ms);
+        }
         pn1_energy_t e;
         if ((rc = pn1_energy(ctx, &e)) != PN1_OK) die_ctx("pn1_energy", ctx, rc);
         init_ene = e.total;

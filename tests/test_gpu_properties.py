@@ -159,9 +159,11 @@ def test_sticky9_end_to_end(tmp_path):
     follow the oracle's trajectory and energies (.dat head, LS9 masses)."""
     out_l, snap_l = run_sticky9(tmp_path, "legacy", "--legacy")
     out_r, snap_r = run_sticky9(tmp_path, "resident")
+    _, snap_a = run_sticky9(tmp_path, "autotuned", "--autotune")
     assert out_l.splitlines()[:2] == ["Init ", "start "] and out_l.endswith("0.005859 \n")
     assert snap_l.shape == (5, 130, 3) and snap_r.shape == snap_l.shape  # t0, 3 outputs, end
     assert np.abs(snap_l - snap_r).max() <= 1e-6
+    assert np.abs(snap_a - snap_r).max() <= 1e-6
     d = load_fixture("dat_n65")
     x, v, m = d["x"], d["v"], d["m"]
     want, e_or = [np.vstack([x, v])], []
