@@ -249,3 +249,40 @@ def test_ens_autotune_keeps_results():
         a1 = ens.force()
     for p, q in zip(a0, a1):
         assert np.abs(p - q).max() <= 1e-13 * np.abs(p).max()
+
+
+def test_gpu_matches_oracle_on_random_small_systems():
+    """hypothesis draws small systems (n = 1 ... 40, random masses, eps, c,
+    BH off-centre and moving): FP64 forces and energies of the HIP path
+    against the oracle."""
+    from hypothesis import given, settings, strategies as st
+
+    ctx = pn.Context(capacity=64)
+
+    @settings(max_examples=40, deadline=None)
+    @given(st.integers(1, 40), st.integers(0, 2**31 - 1), st.sampled_from([1e-3, 1e-2, 0.1]),
+           st.sampled_from([1.0, 3.0, 100.0]))
+    def check(n, seed, eps, c):
+        rng = np.random.default_rng(seed)
+        x = rng.uniform(-1, 1, (n, 3))
+        v = rng.normal(0, 0.5, (n, 3))
+        m = rng.uniform(0.01, 1.0, n)
+        d = np.linalg.norm(x[1:] - x[0], axis=1)
+        x[1:][d < 1e-3] += 0.01
+        ctx.set_params(eps, c, 2.0**-10, pn.FP64)
+        ctx.upload(x, v, m)
+        a = ctx.force()
+        ref = ob.orc_force(x, v, m, eps, c)
+        scale = np.abs(ref).max() if n > 1 else 1.0
+        num = np.linalg.norm(a - ref, axis=1)
+        den = np.maximum(np.linalg.norm(ref, axis=1), 1e-3 * scale)
+        assert (num <= 1e-11 * den).all(), (n, seed, eps, c, (num / den).max())
+        e = ctx.energy()
+        er = ob.orc_energy_terms(x, v, m, eps, c)
+        assert np.abs(e - er).max() <= 1e-11 * max(1.0, np.abs(er).max()) + \
+            4e-15 * float((m * m).sum()) / eps, (n, seed, e, er)
+
+    try:
+        check()
+    finally:
+        ctx.close()
