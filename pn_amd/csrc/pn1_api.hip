@@ -1,5 +1,6 @@
-// pn1_api.hip -- host side of libpn1: the context API and the three
-// legacy shapes of Limit-sticky9 (force, energy, initial_energy).
+// pn1_api.hip -- host side of libpn1: the context API, the three legacy
+// shapes of Limit-sticky9 (force, energy, initial_energy), the autotuner and,
+// through pn1_ens.inc, the ensemble API.  The adjoint is in pn1_adj.hip.
 //
 // Device layout (FP64 SoA, stride cap, particle 0 = BH):
 //   X[6][cap]   state at the start of a step (x, y, z, vx, vy, vz)
@@ -9,8 +10,10 @@
 //   j64[cap] / j32[cap]   per-evaluation j-records (AoS)
 //   part        pair partials: FP64 [S][16][cap]; mixed [S][16 + 3][cap] floats
 //               (the sums, then the Kahan compensations of the Newtonian sums)
-// Nothing is allocated after pn1_upload / pn1_set_tuning: pn1_force,
-// pn1_energy and pn1_step only launch kernels and copy results.
+// Device memory is allocated by pn1_create, and grown by pn1_upload,
+// pn1_set_tuning and pn1_autotune (the partial sums follow the j split) and
+// by the first adjoint call; pn1_force, pn1_energy and pn1_step only launch
+// kernels and copy results.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
