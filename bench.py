@@ -88,11 +88,34 @@ def run_form(ctx, x, v, m, mode, steps, warmup, autotune=True):
     )
 
 
-def roofline(n, avg_ms, peak, traffic=None):
+def roofline(n, avg_ms, peak, traffic=None, valu=None):
     flops = FLOPS_PER_INTERACTION * float(n) * n
     achieved = flops / (avg_ms * 1e-3) / 1e12
-    return {"bound": "valu", "achieved": round(achieved, 3), "peak": peak / 1e12,
-            "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4), "traffic": traffic}
+    out = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak / 1e12,
+           "unit": "TFLOP/s", "frac": round(achieved * 1e12 / peak, 4), "traffic": traffic}
+    if valu:
+        # the FLOPs of the VALU instructions the kernel issues (rocprofv3 counters,
+        # profiles/pair_kernel_valu_n131073.txt), beside the algorithmic figure
+        issued = valu["valu_flops_f32"] + valu["valu_flops_f64"]
+        out["valu_counters"] = {"tflops": round(issued / (avg_ms * 1e-3) / 1e12, 3),
+                                "flops_per_interaction": round(issued / (float(n) * n), 2),
+                                "instructions_per_interaction":
+                                    round(64.0 * valu["valu_wave_instructions"] / (float(n) * n), 2)}
+    return out
+
+
+def load_valu(kind, shape):
+    """Measured VALU work per launch (rocprofv3 SQ_INSTS_VALU_* counters), if
+    profiled at this launch shape."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "pair_kernel_valu_n131073.json")) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    want = d.get("shape", {}).get(kind, {})
+    if any(shape.get(k) != v for k, v in want.items()):
+        return None
+    return d.get(kind)
 
 
 def load_traffic(kind, name="pair_kernel_traffic_n131073.json", shape=None):
@@ -419,12 +442,16 @@ def main():
                    "shape_chosen_by": "pn1_autotune" if tune else "fixed / automatic plan"},
         "roofline": roofline(n, mixed["pair_avg_ms"], PEAK_FP32,
                              load_traffic("mixed", shape={"jsplit": jsplit, "variant": variant,
-                                                          "tile": tile})),
+                                                          "tile": tile}),
+                             load_valu("mixed", {"jsplit": jsplit, "variant": variant,
+                                                 "tile": tile})),
         "fp64": {"interactions_per_s": fp64["interactions_per_s"],
                  "ms_per_step": fp64["ms_per_step"],
                  "roofline": roofline(n, fp64["pair_avg_ms"], PEAK_FP64,
                                       load_traffic("fp64", shape={"jsplit": fp64["shape"][0],
-                                                                  "variant": fp64["shape"][2]})),
+                                                                  "variant": fp64["shape"][2]}),
+                                      load_valu("fp64", {"jsplit": fp64["shape"][0],
+                                                         "variant": fp64["shape"][2]})),
                  "energy_drift": fp64["energy_drift"]},
         "mixed_over_fp64": mixed["interactions_per_s"] / fp64["interactions_per_s"],
         "mixed_accuracy_vs_fp64": {"star_rel_err_max": float(err[1:].max()),
