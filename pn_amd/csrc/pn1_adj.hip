@@ -26,7 +26,7 @@
 #include <cstring>
 #include <vector>
 
-#include "pn1_adj_kernels.hpp"
+#include "pn1_adj_rev.hpp"
 #include "pn1_ctx.hpp"
 #include "pn1_ens.hpp"
 
@@ -38,6 +38,8 @@ struct pn1_adj {
     double *mu = nullptr, *out = nullptr, *L = nullptr, *LY1 = nullptr, *LY2 = nullptr;
     double *T = nullptr, *blk = nullptr, *gm0 = nullptr;
     ARec* rec = nullptr;
+    double* rpart = nullptr;    // pair kernel partials, rpart_n doubles
+    size_t rpart_n = 0;
     double* ckpt = nullptr;     // nck states of 6 x cap
     double* seg = nullptr;      // interval states of 6 x cap
     int nck_cap = 0, seg_cap = 0;
@@ -53,8 +55,8 @@ void adj_free(pn1_ctx* c)
 {
     pn1_adj* a = c ? c->adj : nullptr;
     if (!a) return;
-    void* bufs[] = {a->mu, a->out, a->L, a->LY1, a->LY2, a->T, a->blk, a->gm0, a->rec, a->ckpt,
-                    a->seg};
+    void* bufs[] = {a->mu, a->out, a->L, a->LY1, a->LY2, a->T, a->blk, a->gm0, a->rec, a->rpart,
+                    a->ckpt, a->seg};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete a;
@@ -105,23 +107,51 @@ int adj_get(pn1_ctx* c, pn1_adj** out)
 
 int nblk_of(const pn1_ctx* c) { return c->n > 1 ? (c->n - 1 + WG - 1) / WG : 1; }
 
+// j chunk of the single-pass pair kernel for nst stars in nIb blocks of
+// 256: a multiple of 64 records, as long as still gives about
+// PN1_ADJ_BLOCKS workgroups (several rounds of the whole device), within
+// [128, 2048].  The partials are (nst / L) * 7 + nIb * 6 rows of cap.
+#ifndef PN1_ADJ_BLOCKS
+#define PN1_ADJ_BLOCKS 2048
+#endif
+int adj_chunk(int nst, int nIb)
+{
+    const int s0 = std::max(1, (PN1_ADJ_BLOCKS + nIb - 1) / std::max(nIb, 1));
+    int L = (nst + s0 - 1) / s0;
+    L = (L + 63) / 64 * 64;
+    return std::min(std::max(L, 128), 2048);
+}
+
 // out = J_F(y)^T mu for the state y (SoA 6 x cap); d/dm0 to gm0[1], added
 // to gm0[0].  FP64.
 int enqueue_vjp(pn1_ctx* c, pn1_adj* a, const double* y, const double* mu, double* out,
                 const Phys& P)
 {
     const int nblk = nblk_of(c);
+    int S = 0, nIb = 0;
     if (c->n > 1) {
+        // an explicit j split (pn1_set_tuning, pn1_autotune) sets the chunk
+        // count, rounded to whole tiles; otherwise the VJP picks its own
+        const int nst = c->n - 1;
+        const int L = c->jsplit_req > 0 ? ((nst + c->S - 1) / c->S + 63) / 64 * 64
+                                        : adj_chunk(nst, c->nIblk);
+        S = (nst + L - 1) / L;
+        nIb = c->nIblk;
+        const size_t need = ((size_t)S * NADJ1 + (size_t)nIb * NADJ2) * (size_t)c->cap;
+        if (need > a->rpart_n) {
+            if (a->rpart) (void)hipFree(a->rpart);
+            a->rpart = nullptr;
+            a->rpart_n = 0;
+            int rc = aalloc(c, &a->rpart, need);
+            if (rc) return rc;
+            a->rpart_n = need;
+        }
         hipLaunchKernelGGL(k_adj_prep, dim3(nblk), dim3(WG), 0, c->st, y, mu, c->m, P, a->rec);
-        const dim3 grid(c->pair_grid), blk(WG);
-        double* part = (double*)c->part;
-        hipLaunchKernelGGL(k_adj_pair<1>, grid, blk, 0, c->st, a->rec, y, mu, c->m, part, P,
-                           c->S, c->nIblk);
-        hipLaunchKernelGGL(k_adj_pair<2>, grid, blk, 0, c->st, a->rec, y, mu, c->m, part, P,
-                           c->S, c->nIblk);
+        hipLaunchKernelGGL(k_adj_rev, dim3(S * nIb), dim3(WG), 0, c->st, a->rec, y, mu, c->m,
+                           a->rpart, P, L, S);
     }
     hipLaunchKernelGGL(k_adj_fin, dim3(nblk), dim3(WG), 0, c->st, a->rec, y, mu, c->m,
-                       (const double*)c->part, c->S, out, a->blk, P);
+                       (const double*)a->rpart, S, nIb, out, a->blk, P);
     hipLaunchKernelGGL(k_adj_bh, dim3(1), dim3(WG), 0, c->st, a->blk, nblk, mu, out, a->gm0, P);
     HIPCHK(c, hipGetLastError());
     return PN1_OK;
@@ -299,9 +329,9 @@ int pn1_adjoint_backward(pn1_ctx* c, const double* gx, const double* gv, double*
 
 // =========================================================================
 // Ensembles (pn1_ens_force_vjp, pn1_ens_adjoint_*): the same VJP and stage
-// adjoint for every cluster at once.  The two pair passes are persistent
-// launches over the ensemble's own work items (cluster, 256-star block,
-// j chunk) and per-XCD queues; each item writes its own partial slots.
+// adjoint for every cluster at once.  The single-pass pair kernel is a
+// persistent launch over the ensemble's own work items (cluster, 256-star
+// block, j chunk) and per-XCD queues; each item writes its own partial rows.
 // =========================================================================
 namespace pn1dev {
 
@@ -323,40 +353,61 @@ __global__ void k_ens_adj_reset(unsigned int* __restrict__ counter)
     if (threadIdx.x < NXCD) counter[threadIdx.x] = 0u;
 }
 
-template <int PASS>
-__global__ __launch_bounds__(WG) void k_ens_adj_pair(const ARec* __restrict__ rec,
-                                                    const double* __restrict__ y,
-                                                    const double* __restrict__ mu,
-                                                    const double* __restrict__ m,
-                                                    double* __restrict__ part,
-                                                    const EItem* __restrict__ items,
-                                                    unsigned int* __restrict__ counter,
-                                                    const int* __restrict__ qoff, Phys P)
+// Persistent single-pass pair kernel over the ensemble's work items.  Item
+// (cluster c, block ib, chunk s): stars off + 1 + ib*WG + lane, records
+// [j0, j1).  The partials of cluster c start at apart + aoff[c], rows of
+// stride nst: i side (s*NADJ1 + k), s < S_c; then j side
+// (S_c*NADJ1 + ib*NADJ2 + k).  Every item writes its own rows.
+__global__ __launch_bounds__(WG) void k_ens_adj_rev(const ARec* __restrict__ rec,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ mu,
+                                                   const double* __restrict__ m,
+                                                   double* __restrict__ apart,
+                                                   const long long* __restrict__ aoff,
+                                                   const EItem* __restrict__ items,
+                                                   unsigned int* __restrict__ counter,
+                                                   const int* __restrict__ qoff, Phys P, int L)
 {
+    __shared__ double sh[NADJ2][4][64];
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     int q = 0;
     for (;;) {
         const int it = ens_take(counter, qoff, q);
         if (it < 0) break;
         const EItem w = items[it];
+        const int nst = w.nst;
         const int li = w.ib * WG + (int)threadIdx.x;
-        const int i = w.off + 1 + (li < w.nst ? li : w.nst - 1);
+        const bool iok = li < nst;
+        const int i = w.off + 1 + (iok ? li : nst - 1);
         const ABH B = load_abh(y, mu, m, P.cap, w.off);
+        const RevU U = rev_uniform(B, P.eps2, P.lam);
         const ARec me = rec[i];
-        double* pp = part + w.pslot * NADJ + li;
-        if (PASS == 1) {
-            double acc[NADJ1];
-            adj_pass1(rec, B, me, i, w.j0, w.j1, P.eps2, P.lam, acc);
-            if (li < w.nst) {
+        const RevI I = rev_setup(me, U);
+        RevIAcc ai;
+        rev_zero(ai);
+        const int S = ens_nchunks(nst, L), s = (w.j0 - w.off - 1) / L;
+        double* base = apart + aoff[w.c];
+        double* pj = base + (size_t)S * NADJ1 * nst;
+        for (int jt = w.j0; jt < w.j1; jt += 64) {
+            double g[NADJ2];
+            rev_tile(rec, I, U, i, iok, jt, w.j1, ai, g);
 #pragma unroll
-                for (int k = 0; k < NADJ1; ++k) pp[(size_t)k * w.nst] = acc[k];
-            }
-        } else {
-            double acc[NADJ2];
-            adj_pass2(rec, B, me, i, w.j0, w.j1, P.eps2, P.lam, acc);
-            if (li < w.nst) {
+            for (int k = 0; k < NADJ2; ++k) sh[k][wv][lane] = g[k];
+            __syncthreads();
+            const int jl = jt + lane;
+            if (wv == 0 && jl < w.j1) {
 #pragma unroll
-                for (int k = 0; k < NADJ2; ++k) pp[(size_t)(NADJ1 + k) * w.nst] = acc[k];
+                for (int k = 0; k < NADJ2; ++k)
+                    pj[((size_t)w.ib * NADJ2 + k) * nst + (jl - w.off - 1)] =
+                        ((sh[k][0][lane] + sh[k][1][lane]) + sh[k][2][lane]) + sh[k][3][lane];
             }
+            __syncthreads();
+        }
+        double g[NADJ1];
+        rev_ifin(me, U, ai, g);
+        if (iok) {
+#pragma unroll
+            for (int k = 0; k < NADJ1; ++k) base[((size_t)s * NADJ1 + k) * nst + li] = g[k];
         }
     }
 }
@@ -367,8 +418,8 @@ __global__ __launch_bounds__(WG) void k_ens_adj_fin(const ARec* __restrict__ rec
                                                    const double* __restrict__ m,
                                                    const int* __restrict__ coff,
                                                    const EBlk* __restrict__ blk,
-                                                   const double* __restrict__ part,
-                                                   const long long* __restrict__ poff,
+                                                   const double* __restrict__ apart,
+                                                   const long long* __restrict__ aoff,
                                                    double* __restrict__ out,
                                                    double* __restrict__ ablk, Phys P, int L)
 {
@@ -381,13 +432,16 @@ __global__ __launch_bounds__(WG) void k_ens_adj_fin(const ARec* __restrict__ rec
         const ABH B = load_abh(y, mu, m, P.cap, off);
         double g[NADJ1];
         adj_star_grad(rec[p], B, P.eps, P.eps2, P.lam, g);
-        const double* base = part + poff[b.c] * NADJ + li;
-        const int S = ens_nchunks(nst, L);
+        const double* base = apart + aoff[b.c] + li;
+        const int S = ens_nchunks(nst, L), nIb = (nst + WG - 1) / WG;
         for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int k = 0; k < NADJ1; ++k) g[k] += base[(size_t)(s * NADJ + k) * nst];
+            for (int k = 0; k < NADJ1; ++k) g[k] += base[(size_t)(s * NADJ1 + k) * nst];
+        }
+        const double* pj = base + (size_t)S * NADJ1 * nst;
+        for (int ib = 0; ib < nIb; ++ib) {
 #pragma unroll
-            for (int k = 0; k < NADJ2; ++k) g[k] += base[(size_t)(s * NADJ + NADJ1 + k) * nst];
+            for (int k = 0; k < NADJ2; ++k) g[k] += pj[(size_t)(ib * NADJ2 + k) * nst];
         }
         adj_store(g, mu, out, (size_t)P.cap, p, t);
     }
@@ -432,7 +486,12 @@ struct pn1_ens_adj {
     double* ckpt = nullptr;
     double* seg = nullptr;
     int nck_cap = 0, seg_cap = 0, ablk_cap = 0;
-    int grid1 = 0, grid2 = 0;
+    int grid = 0;                 // resident workgroups of k_ens_adj_rev
+    double* apart = nullptr;      // pair kernel partials, apart_n doubles
+    size_t apart_n = 0;
+    long long* aoff = nullptr;    // per-cluster offsets into apart (capc)
+    unsigned tab_gen = 0;         // (gen, L) the offsets were made for
+    int tab_L = -1;
     bool have_fwd = false;
     int steps = 0, interval = 0;
     unsigned gen = 0;
@@ -445,7 +504,7 @@ void ens_adj_free(pn1_ens* e)
     pn1_ens_adj* a = e ? e->adj : nullptr;
     if (!a) return;
     void* bufs[] = {a->mu, a->out, a->L, a->LY1, a->LY2, a->T, a->ablk, a->gm0, a->rec,
-                    a->ckpt, a->seg};
+                    a->ckpt, a->seg, a->apart, a->aoff};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     delete a;
@@ -493,8 +552,8 @@ int ens_adj_get(pn1_ens* e, pn1_ens_adj** out)
         if (!rc) rc = ealloc_adj(e, &a->T, s6);
         if (!rc) rc = ealloc_adj(e, &a->gm0, 2 * (size_t)e->capc);
         if (!rc) rc = ealloc_adj(e, &a->rec, (size_t)e->cap + 2);
-        if (!rc) rc = adj_resident_grid(e, k_ens_adj_pair<1>, &a->grid1);
-        if (!rc) rc = adj_resident_grid(e, k_ens_adj_pair<2>, &a->grid2);
+        if (!rc) rc = ealloc_adj(e, &a->aoff, (size_t)e->capc);
+        if (!rc) rc = adj_resident_grid(e, k_ens_adj_rev, &a->grid);
         if (rc) {
             ens_adj_free(e);
             return rc;
@@ -509,6 +568,31 @@ int ens_adj_get(pn1_ens* e, pn1_ens_adj** out)
         if (rc) return rc;
         a->ablk_cap = e->nb;
     }
+    if (a->tab_L != e->L || a->tab_gen != e->gen) {   // partial layout of the clusters
+        std::vector<long long> aoff(e->nc > 0 ? e->nc : 1, 0);
+        size_t tot = 0;
+        for (int c = 0; c < e->nc; ++c) {
+            const int nst = e->off_h[c + 1] - e->off_h[c] - 1;
+            aoff[c] = (long long)tot;
+            if (nst > 0)
+                tot += ((size_t)((nst + e->L - 1) / e->L) * NADJ1 +
+                        (size_t)((nst + WG - 1) / WG) * NADJ2) * (size_t)nst;
+        }
+        if (tot > a->apart_n) {
+            if (a->apart) (void)hipFree(a->apart);
+            a->apart = nullptr;
+            a->apart_n = 0;
+            int rc = ealloc_adj(e, &a->apart, tot);
+            if (rc) return rc;
+            a->apart_n = tot;
+        }
+        if (e->nc > 0)
+            EHIPCHK(e, hipMemcpyAsync(a->aoff, aoff.data(), e->nc * sizeof(long long),
+                                      hipMemcpyHostToDevice, e->st));
+        EHIPCHK(e, hipStreamSynchronize(e->st));
+        a->tab_L = e->L;
+        a->tab_gen = e->gen;
+    }
     *out = a;
     return PN1_OK;
 }
@@ -520,17 +604,14 @@ int ens_enqueue_vjp(pn1_ens* e, pn1_ens_adj* a, const double* y, const double* m
         hipLaunchKernelGGL(k_ens_adj_prep, dim3(e->nb), dim3(WG), 0, e->st, y, mu, e->m, e->coff,
                            e->blk, P, a->rec);
     if (e->nitems) {
-        double* part = (double*)e->part;
         hipLaunchKernelGGL(k_ens_adj_reset, dim3(1), dim3(64), 0, e->st, e->counter);
-        hipLaunchKernelGGL(k_ens_adj_pair<1>, dim3(std::min(a->grid1, e->nitems)), dim3(WG), 0,
-                           e->st, a->rec, y, mu, e->m, part, e->items, e->counter, e->qoff, P);
-        hipLaunchKernelGGL(k_ens_adj_reset, dim3(1), dim3(64), 0, e->st, e->counter);
-        hipLaunchKernelGGL(k_ens_adj_pair<2>, dim3(std::min(a->grid2, e->nitems)), dim3(WG), 0,
-                           e->st, a->rec, y, mu, e->m, part, e->items, e->counter, e->qoff, P);
+        hipLaunchKernelGGL(k_ens_adj_rev, dim3(std::min(a->grid, e->nitems)), dim3(WG), 0, e->st,
+                           a->rec, y, mu, e->m, a->apart, a->aoff, e->items, e->counter, e->qoff,
+                           P, e->L);
     }
     if (e->nb)
         hipLaunchKernelGGL(k_ens_adj_fin, dim3(e->nb), dim3(WG), 0, e->st, a->rec, y, mu, e->m,
-                           e->coff, e->blk, (const double*)e->part, e->poff, out, a->ablk, P,
+                           e->coff, e->blk, (const double*)a->apart, a->aoff, out, a->ablk, P,
                            e->L);
     hipLaunchKernelGGL(k_ens_adj_bh, dim3(e->nc), dim3(WG), 0, e->st, a->ablk, e->cblk, e->coff,
                        mu, out, a->gm0, P);

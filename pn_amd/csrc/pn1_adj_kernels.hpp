@@ -12,30 +12,29 @@
 //               star's BH 1PN, aa_cross and self-removal terms
 //               (Limit-sticky9.c:54-115, 179-274, 386-422)
 // Both are written once below as templates over the number type of each
-// side; the kernels instantiate them with forward-mode duals seeded on the
-// lane's own coordinates, so the dual's tangents are the gradient.
-//   pass 1  lanes i, j loop:   d phi_ij / d(x_i, v_i, m0), summed over j != i
-//   pass 2  lanes j, i loop:   d phi_ij / d(x_j, v_j), summed over i != j
-//   fin     lanes i:           d (psi_i + phi_ii) / d(x_i, v_i, m0), plus
-//                              both passes
+// side.  The pair sums run in pn1_adj_rev.hpp: one hand-written reverse
+// sweep of phi_ij per pair gives d/d(x_i, v_i, m0) and d/d(x_j, v_j)
+// together (pairs j = i skipped).  The per-star terms run here with
+// forward-mode duals seeded on the lane's own coordinates (the dual's
+// tangents are the gradient):
+//   fin     lanes i:   d (psi_i + phi_ii) / d(x_i, v_i, m0), plus the pair
+//                      kernel's partial sums
 // The self pair phi_ii is differentiated as one function of particle i (both
 // sides carry the same duals, so r = x_i - x_i is zero with zero tangents):
-// split over the two passes its partial derivatives would be +-m/eps^3
+// split into its i and j sides its partial derivatives would be +-m/eps^3
 // terms cancelling to rounding.
 // Every term depends on positions only through differences, so Phi is
 // invariant under a common shift and the BH's position gradient is minus
-// the sum of the stars'; no term depends on the BH's velocity.  The
-// partial sums of both passes use the j split of the force kernels (fixed
-// order), so results are bitwise reproducible.
+// the sum of the stars'; no term depends on the BH's velocity.  All partial
+// sums are added in a fixed order, so results are bitwise reproducible.
 #pragma once
 #include "pn1_common.hpp"
 #include "pn1_dual.hpp"
 
 namespace pn1dev {
 
-constexpr int NADJ1 = 7;              // pass 1: x_i, v_i, m0
-constexpr int NADJ2 = 6;              // pass 2: x_j, v_j
-constexpr int NADJ = NADJ1 + NADJ2;   // partial rows per chunk
+constexpr int NADJ1 = 7;              // i side: x_i, v_i, m0
+constexpr int NADJ2 = 6;              // j side: x_j, v_j
 
 // Per-star adjoint record (FP64, 128 bytes): state, cotangent u, mass, BH
 // geometry d = x - x_BH, D2 = |d|^2, e = (D2 + eps^2)^-1/2.
@@ -65,7 +64,7 @@ __device__ __forceinline__ ABH load_abh(const double* __restrict__ st,
 }
 
 template <class A, class B>
-__device__ __forceinline__ auto dot(const A (&a)[3], const B (&b)[3])
+__host__ __device__ __forceinline__ auto dot(const A (&a)[3], const B (&b)[3])
 {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
@@ -75,7 +74,7 @@ __device__ __forceinline__ auto dot(const A (&a)[3], const B (&b)[3])
 // cotangents u_i and u_0.  ac uses b = x_0 - x_i = -d_i, c = -d_j; a1c uses
 // d_i, d_j; both use r = x_j - x_i.  1/(|r|^2 + eps^2) is written w^2.
 template <class TI, class TJ, class TM>
-__device__ __forceinline__ auto adj_pair_phi(const TI (&xi)[3], const TI (&vi)[3],
+__host__ __device__ __forceinline__ auto adj_pair_phi(const TI (&xi)[3], const TI (&vi)[3],
                                              const TI (&di)[3], const TI& rb2, const TI& rbe,
                                              const TI& rbe3, const TI& vvi,
                                              const TJ (&xj)[3], const TJ (&vj)[3],
@@ -135,7 +134,7 @@ __device__ __forceinline__ auto adj_pair_phi(const TI (&xi)[3], const TI (&vi)[3
 
 // psi_i for star i: d_i = x_i - x_0, |d_i|^2, e_i; velocity; masses.
 template <class T, class TM>
-__device__ __forceinline__ auto adj_star_psi(const T (&vi)[3], const T (&di)[3], const T& rb2,
+__host__ __device__ __forceinline__ auto adj_star_psi(const T (&vi)[3], const T (&di)[3], const T& rb2,
                                              const T& rbe, double mi, const TM& m0,
                                              const TM& im0, const double (&ui)[3],
                                              const double (&u0)[3], double eps, double lam)
@@ -200,71 +199,6 @@ __global__ __launch_bounds__(WG) void k_adj_prep(const double* __restrict__ y,
     if (p < P.n) adj_prep_star(y, mu, m, (size_t)P.cap, P.eps2, p, 0, rec);
 }
 
-// Pass 1 over the records [j0, j1) for the lane's star i (j = i skipped):
-// d/d(x_i, v_i, m0) of sum_j phi_ij.
-__device__ __forceinline__ void adj_pass1(const ARec* __restrict__ rec, const ABH& B,
-                                          const ARec& me, int i, int j0, int j1, double eps2,
-                                          double lam, double (&acc)[NADJ1])
-{
-    using D = Dual<NADJ1>;
-    const D xi[3] = {dseed<NADJ1>(me.x, 0), dseed<NADJ1>(me.y, 1), dseed<NADJ1>(me.z, 2)};
-    const D vi[3] = {dseed<NADJ1>(me.vx, 3), dseed<NADJ1>(me.vy, 4), dseed<NADJ1>(me.vz, 5)};
-    const D m0 = dseed<NADJ1>(B.m0, 6);
-    const D im0 = dinv(m0);
-    const D di[3] = {xi[0] - B.x0[0], xi[1] - B.x0[1], xi[2] - B.x0[2]};
-    const D rb2 = dot(di, di);
-    const D rbe = drsq(rb2 + eps2);
-    const D rbe3 = rbe * rbe * rbe;
-    const D vvi = dot(vi, vi);
-    const double ume[3] = {me.ux, me.uy, me.uz};
-#pragma unroll
-    for (int k = 0; k < NADJ1; ++k) acc[k] = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        const ARec J = rec[j];
-        const double xj[3] = {J.x, J.y, J.z}, vj[3] = {J.vx, J.vy, J.vz};
-        const double dj[3] = {J.dx, J.dy, J.dz};
-        const double rce3 = J.e * J.e * J.e;
-        const D phi = adj_pair_phi(xi, vi, di, rb2, rbe, rbe3, vvi, xj, vj, dj, J.D2, J.e, rce3,
-                                   me.m, J.m, m0, im0, ume, B.u0, eps2, lam);
-        if (j != i) {
-#pragma unroll
-            for (int k = 0; k < NADJ1; ++k) acc[k] += phi.d[k];
-        }
-    }
-}
-
-// Pass 2 over the records [i0, i1) on the i side, for the lane's star j
-// (i = j skipped): d/d(x_j, v_j) of sum_i phi_ij.
-__device__ __forceinline__ void adj_pass2(const ARec* __restrict__ rec, const ABH& B,
-                                          const ARec& me, int jme, int i0, int i1, double eps2,
-                                          double lam, double (&acc)[NADJ2])
-{
-    using D = Dual<NADJ2>;
-    const D xj[3] = {dseed<NADJ2>(me.x, 0), dseed<NADJ2>(me.y, 1), dseed<NADJ2>(me.z, 2)};
-    const D vj[3] = {dseed<NADJ2>(me.vx, 3), dseed<NADJ2>(me.vy, 4), dseed<NADJ2>(me.vz, 5)};
-    const D dj[3] = {xj[0] - B.x0[0], xj[1] - B.x0[1], xj[2] - B.x0[2]};
-    const D rc2 = dot(dj, dj);
-    const D rce = drsq(rc2 + eps2);
-    const D rce3 = rce * rce * rce;
-    const double im0 = 1.0 / B.m0;
-#pragma unroll
-    for (int k = 0; k < NADJ2; ++k) acc[k] = 0.0;
-    for (int i = i0; i < i1; ++i) {
-        const ARec I = rec[i];
-        const double xi[3] = {I.x, I.y, I.z}, vi[3] = {I.vx, I.vy, I.vz};
-        const double di[3] = {I.dx, I.dy, I.dz};
-        const double ui[3] = {I.ux, I.uy, I.uz};
-        const double rbe3 = I.e * I.e * I.e;
-        const double vvi = I.vx * I.vx + I.vy * I.vy + I.vz * I.vz;
-        const D phi = adj_pair_phi(xi, vi, di, I.D2, I.e, rbe3, vvi, xj, vj, dj, rc2, rce, rce3,
-                                   I.m, me.m, B.m0, im0, ui, B.u0, eps2, lam);
-        if (i != jme) {
-#pragma unroll
-            for (int k = 0; k < NADJ2; ++k) acc[k] += phi.d[k];
-        }
-    }
-}
-
 // d (psi_i + phi_ii) / d(x_i, v_i, m0) for star i with record `me`.  The
 // self pair phi_ii is taken with both sides on the same variables: r = 0
 // and its tangents cancel exactly, as in the reference's own sum.
@@ -303,51 +237,16 @@ __device__ __forceinline__ void adj_store(const double (&g)[NADJ1], const double
     t[3] = g[6];
 }
 
-// Pair passes.  Partials: part[(s * NADJ + row) * cap + p], rows 0..6 from
-// pass 1, 7..12 from pass 2.
-template <int PASS>
-__global__ __launch_bounds__(WG) void k_adj_pair(const ARec* __restrict__ rec,
-                                                const double* __restrict__ y,
-                                                const double* __restrict__ mu,
-                                                const double* __restrict__ m,
-                                                double* __restrict__ part, Phys P, int S,
-                                                int nIblk)
-{
-    int s, ib;
-    map_block(blockIdx.x, S, s, ib);
-    if (ib >= nIblk) return;
-    const int n = P.n;
-    const int li = ib * WG + threadIdx.x + 1;
-    const int i = li < n ? li : n - 1;
-    const ABH B = load_abh(y, mu, m, P.cap, 0);
-    const JRange R = j_range(s, S, n, ib);
-    const ARec me = rec[i];
-    if (PASS == 1) {
-        double acc[NADJ1];
-        adj_pass1(rec, B, me, i, R.j0, R.j1, P.eps2, P.lam, acc);
-        if (li < n) {
-#pragma unroll
-            for (int k = 0; k < NADJ1; ++k) part[(size_t)(s * NADJ + k) * P.cap + li] = acc[k];
-        }
-    } else {
-        double acc[NADJ2];
-        adj_pass2(rec, B, me, i, R.j0, R.j1, P.eps2, P.lam, acc);
-        if (li < n) {
-#pragma unroll
-            for (int k = 0; k < NADJ2; ++k)
-                part[(size_t)(s * NADJ + NADJ1 + k) * P.cap + li] = acc[k];
-        }
-    }
-}
-
-// Per star: both passes' partials plus the star terms.  Writes
+// Per star: the pair kernel's partials plus the star terms.  part holds S
+// rows of NADJ1 i-side sums (one per j chunk) and then nIb rows of NADJ2
+// j-side sums (one per 256-star i block), each row of stride cap.  Writes
 // out_x = g_x and out_v = mu_x + g_v (the VJP of the stage function
 // (x, v) -> (v, a)), and per-block partials {sum g_x, sum d/dm0}.
 __global__ __launch_bounds__(WG) void k_adj_fin(const ARec* __restrict__ rec,
                                                const double* __restrict__ y,
                                                const double* __restrict__ mu,
                                                const double* __restrict__ m,
-                                               const double* __restrict__ part, int S,
+                                               const double* __restrict__ part, int S, int nIb,
                                                double* __restrict__ out,
                                                double* __restrict__ blk, Phys P)
 {
@@ -360,9 +259,12 @@ __global__ __launch_bounds__(WG) void k_adj_fin(const ARec* __restrict__ rec,
         adj_star_grad(rec[p], B, P.eps, P.eps2, P.lam, g);
         for (int s = 0; s < S; ++s) {
 #pragma unroll
-            for (int k = 0; k < NADJ1; ++k) g[k] += part[(size_t)(s * NADJ + k) * cap + p];
+            for (int k = 0; k < NADJ1; ++k) g[k] += part[(size_t)(s * NADJ1 + k) * cap + p];
+        }
+        const double* pj = part + (size_t)S * NADJ1 * cap;
+        for (int b = 0; b < nIb; ++b) {
 #pragma unroll
-            for (int k = 0; k < NADJ2; ++k) g[k] += part[(size_t)(s * NADJ + NADJ1 + k) * cap + p];
+            for (int k = 0; k < NADJ2; ++k) g[k] += pj[(size_t)(b * NADJ2 + k) * cap + p];
         }
         adj_store(g, mu, out, cap, p, t);
     }

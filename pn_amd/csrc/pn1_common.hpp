@@ -73,14 +73,19 @@ struct Phys {
 // FP64 reciprocal square root.  v_rsq_f64 alone is good to ~2^-24
 // relative (measured on gfx950), so one correction of third order follows:
 // with r = 1 - x y^2, 1/sqrt(x) = y (1 - r)^(-1/2) = y (1 + r/2 + 3r^2/8 + ...),
-// truncation ~ (5/16) r^3 ~ 1e-22; the result is within ~1 ulp.
-__device__ __forceinline__ double rsq64(double x)
+// truncation ~ (5/16) r^3 ~ 1e-22; the result is within ~1 ulp.  The host
+// pass (CPU checks of the device functions) uses 1 / sqrt.
+__host__ __device__ __forceinline__ double rsq64(double x)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
     const double y = __builtin_amdgcn_rsq(x);
     const double h = x * y;
     const double r = __builtin_fma(-h, y, 1.0);
     const double t = __builtin_fma(0.375, r, 0.5);
     return __builtin_fma(y * r, t, y);
+#else
+    return 1.0 / __builtin_sqrt(x);
+#endif
 }
 
 // Fixed-order workgroup sum of N doubles; thread 0 writes out[0..N).

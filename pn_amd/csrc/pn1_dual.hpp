@@ -22,7 +22,7 @@ struct Dual {
 };
 
 template <int N>
-__device__ __forceinline__ Dual<N> dseed(double v, int k)
+__host__ __device__ __forceinline__ Dual<N> dseed(double v, int k)
 {
     Dual<N> r;
     r.v = v;
@@ -31,13 +31,13 @@ __device__ __forceinline__ Dual<N> dseed(double v, int k)
     return r;
 }
 
-__device__ __forceinline__ double val(double a) { return a; }
+__host__ __device__ __forceinline__ double val(double a) { return a; }
 template <int N>
-__device__ __forceinline__ double val(const Dual<N>& a) { return a.v; }
+__host__ __device__ __forceinline__ double val(const Dual<N>& a) { return a.v; }
 
 // ---- addition / subtraction ----------------------------------------------
 template <int N>
-__device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b)
 {
     Dual<N> r;
     r.v = a.v + b.v;
@@ -46,19 +46,19 @@ __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b)
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, double b)
+__host__ __device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, double b)
 {
     Dual<N> r = a;
     r.v = a.v + b;
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator+(double a, const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> operator+(double a, const Dual<N>& b)
 {
     return b + a;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a)
+__host__ __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a)
 {
     Dual<N> r;
     r.v = -a.v;
@@ -67,7 +67,7 @@ __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a)
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b)
 {
     Dual<N> r;
     r.v = a.v - b.v;
@@ -76,14 +76,14 @@ __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b)
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, double b)
+__host__ __device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, double b)
 {
     Dual<N> r = a;
     r.v = a.v - b;
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator-(double a, const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> operator-(double a, const Dual<N>& b)
 {
     Dual<N> r;
     r.v = a - b.v;
@@ -94,7 +94,7 @@ __device__ __forceinline__ Dual<N> operator-(double a, const Dual<N>& b)
 
 // ---- multiplication / division -------------------------------------------
 template <int N>
-__device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b)
 {
     Dual<N> r;
     r.v = a.v * b.v;
@@ -103,7 +103,7 @@ __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b)
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, double b)
+__host__ __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, double b)
 {
     Dual<N> r;
     r.v = a.v * b;
@@ -112,13 +112,13 @@ __device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, double b)
     return r;
 }
 template <int N>
-__device__ __forceinline__ Dual<N> operator*(double a, const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> operator*(double a, const Dual<N>& b)
 {
     return b * a;
 }
 // 1 / b
 template <int N>
-__device__ __forceinline__ Dual<N> dinv(const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> dinv(const Dual<N>& b)
 {
     Dual<N> r;
     r.v = 1.0 / b.v;
@@ -127,11 +127,11 @@ __device__ __forceinline__ Dual<N> dinv(const Dual<N>& b)
     for (int i = 0; i < N; ++i) r.d[i] = c * b.d[i];
     return r;
 }
-__device__ __forceinline__ double dinv(double b) { return 1.0 / b; }
+__host__ __device__ __forceinline__ double dinv(double b) { return 1.0 / b; }
 
 // b^(-1/2) (b > 0), through the library's FP64 rsqrt
 template <int N>
-__device__ __forceinline__ Dual<N> drsq(const Dual<N>& b)
+__host__ __device__ __forceinline__ Dual<N> drsq(const Dual<N>& b)
 {
     Dual<N> r;
     r.v = rsq64(b.v);
@@ -140,6 +140,6 @@ __device__ __forceinline__ Dual<N> drsq(const Dual<N>& b)
     for (int i = 0; i < N; ++i) r.d[i] = c * b.d[i];
     return r;
 }
-__device__ __forceinline__ double drsq(double b) { return rsq64(b); }
+__host__ __device__ __forceinline__ double drsq(double b) { return rsq64(b); }
 
 }  // namespace pn1dev

@@ -11,9 +11,10 @@ namespace pn1dev {
 //   off    the cluster's BH index; its stars are off + 1 .. off + nst
 //   ib     i-block (local stars ib * 256 ..)
 //   j0, j1 the chunk's particle range
+//   c      the cluster's index
 //   pslot  first partial slot of the chunk: poff[c] + s * nst
 struct __align__(32) EItem {
-    int off, nst, ib, j0, j1, pad;
+    int off, nst, ib, j0, j1, c;
     long long pslot;
 };
 
