@@ -284,12 +284,12 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
 
 
 def run_adjoint(steps, interval, vjp_reps=5):
-    """The differentiation workload: N = 16384 Plummer (seed 8) + BH 0.02,
-    c = 100, eps = 1e-3, dt = 2^-10: `steps` steps forward with a checkpoint
-    every `interval`, the radial-velocity loss at r < 0.2, and the discrete
-    adjoint back to d loss / d (x0, v0, m0).  Also the force VJP alone
-    (pn1_force_vjp, two O(n^2) dual-number passes) against one FP64 force
-    evaluation at the same state."""
+    """The differentiation workload (workloads.ADJOINT: N = 16384 Plummer,
+    seed 4, + BH 0.02, c = 100, eps = 1e-3, dt = 2^-10): `steps` steps forward
+    with a checkpoint every `interval`, the radial-velocity loss at r < 0.2,
+    and the discrete adjoint back to d loss / d (x0, v0, m0).  Also the force
+    VJP alone (pn1_force_vjp: one O(n^2) reverse-sweep pass over both sides of
+    every pair) against one FP64 force evaluation at the same state."""
     W = workloads.ADJOINT
     x, v, m = pn.ic_plummer(W["n_stars"], W["seed"], a=1.0, m_bh=W["m_bh"])
     n = len(m)
