@@ -52,9 +52,10 @@ struct __align__(16) JRec64 {
     double x, y, z, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e;
 };
 // j-record, mixed form: double-single position (hi, lo), the rest rounded
-// to FP32.  80 bytes.
+// to FP32, and Qd = Q d (the pair term Q w d is then one FMA per component).
+// 80 bytes.
 struct __align__(16) JRec32 {
-    float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e, pad;
+    float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, D2, A, B, MB, Qdx, Qdy, Qdz, pad[3];
 };
 static_assert(sizeof(JRec64) == 128, "JRec64 layout");
 static_assert(sizeof(JRec32) == 80, "JRec32 layout");

@@ -98,9 +98,9 @@ __device__ __forceinline__ void prep_star(const double* __restrict__ st,
         h.zl = (float)(z - (double)h.zh);
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
         h.m = (float)mp; h.D2 = (float)D2;
-        h.A = (float)r.A; h.B = (float)r.B; h.Q = (float)r.Q; h.MB = (float)r.MB;
-        h.dx = (float)dx; h.dy = (float)dy; h.dz = (float)dz; h.e = (float)e;
-        h.pad = 0.0f;
+        h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
+        h.Qdx = (float)(r.Q * dx); h.Qdy = (float)(r.Q * dy); h.Qdz = (float)(r.Q * dz);
+        h.pad[0] = h.pad[1] = h.pad[2] = 0.0f;
         j32[p] = h;
     }
     g[G_G1] = mp * e;
@@ -189,10 +189,11 @@ __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64&
     const double t = R2e - I.D2e;
     a[A_ZE] = __builtin_fma(MBj, t, a[A_ZE]);
     const double sr = __builtin_fma(J.sx, rx, __builtin_fma(J.sy, ry, J.sz * rz));
-    double br = __builtin_fma(J.B, t, J.A);
-    br = __builtin_fma(I.hx, J.sx, __builtin_fma(I.hy, J.sy, __builtin_fma(I.hz, J.sz, br)));
+    // one record operand per instruction (one scalar register pair per VALU op)
     const double tv = sr * w;
-    br = __builtin_fma(tv, tv, br);
+    double br = __builtin_fma(tv, tv, J.A);
+    br = __builtin_fma(J.B, t, br);
+    br = __builtin_fma(I.hx, J.sx, __builtin_fma(I.hy, J.sy, __builtin_fma(I.hz, J.sz, br)));
     const double Kr = mw3 * br;
     a[A_RX] = __builtin_fma(Kr, rx, a[A_RX]);
     a[A_RY] = __builtin_fma(Kr, ry, a[A_RY]);
@@ -289,12 +290,14 @@ template <bool MASK, bool RF>
 __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32& I,
                                        float (&a)[NACC], float eps2)
 {
-    float mj = J.m, Qj = J.Q, MBj = J.MB;
+    float mj = J.m, MBj = J.MB, Qdx = J.Qdx, Qdy = J.Qdy, Qdz = J.Qdz;
     if (MASK) {
         const bool self = (j == i);
         mj = self ? 0.0f : mj;
-        Qj = self ? 0.0f : Qj;
         MBj = self ? 0.0f : MBj;
+        Qdx = self ? 0.0f : Qdx;
+        Qdy = self ? 0.0f : Qdy;
+        Qdz = self ? 0.0f : Qdz;
     }
     // separation from the hi and lo parts: close pairs keep their digits
     const float rx = (J.xh - I.xh) + (J.xl - I.xl);
@@ -315,17 +318,18 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
     a[A_XI] = __builtin_fmaf(mw3, I.D2 - J.D2, a[A_XI]);
-    const float qw = Qj * w;
-    a[A_TX] = __builtin_fmaf(qw, J.dx, a[A_TX]);
-    a[A_TY] = __builtin_fmaf(qw, J.dy, a[A_TY]);
-    a[A_TZ] = __builtin_fmaf(qw, J.dz, a[A_TZ]);
+    a[A_TX] = __builtin_fmaf(w, Qdx, a[A_TX]);
+    a[A_TY] = __builtin_fmaf(w, Qdy, a[A_TY]);
+    a[A_TZ] = __builtin_fmaf(w, Qdz, a[A_TZ]);
     const float t = R2e - I.D2e;
     a[A_ZE] = __builtin_fmaf(MBj, t, a[A_ZE]);
     const float sr = __builtin_fmaf(J.sx, rx, __builtin_fmaf(J.sy, ry, J.sz * rz));
-    float br = __builtin_fmaf(J.B, t, J.A);
-    br = __builtin_fmaf(I.hx, J.sx, __builtin_fmaf(I.hy, J.sy, __builtin_fmaf(I.hz, J.sz, br)));
+    // the bracket starts from (s_j.r w)^2 + A_j: one record operand per
+    // instruction (gfx950 reads one scalar register per VALU op), no copy
     const float tv = sr * w;
-    br = __builtin_fmaf(tv, tv, br);
+    float br = __builtin_fmaf(tv, tv, J.A);
+    br = __builtin_fmaf(J.B, t, br);
+    br = __builtin_fmaf(I.hx, J.sx, __builtin_fmaf(I.hy, J.sy, __builtin_fmaf(I.hz, J.sz, br)));
     const float Kr = mw3 * br;
     a[A_RX] = __builtin_fmaf(Kr, rx, a[A_RX]);
     a[A_RY] = __builtin_fmaf(Kr, ry, a[A_RY]);
