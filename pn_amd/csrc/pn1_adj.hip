@@ -1,9 +1,11 @@
 // pn1_adj.hip -- adjoint of libpn1's force and stage driver (the
 // pn1_force_vjp / pn1_adjoint_* entry points).  A translation unit of its
 // own: it is built with -ffinite-math-only -fno-signed-zeros so that the
-// dual-number arithmetic of the VJP kernels drops the zero tangents of its
-// seeds at compile time; the forward kernels keep strict IEEE semantics in
-// pn1_api.hip.
+// dual-number arithmetic of the per-star VJP terms drops the zero tangents
+// of its seeds at compile time; the forward kernels keep strict IEEE
+// semantics in pn1_api.hip.  The O(n^2) pair sums of the VJP run in one
+// pass (k_adj_rev, pn1_adj_rev.hpp): a hand-written reverse sweep per pair
+// gives both of its sides, the j records rotating through the wave's lanes.
 //
 // The stage driver (Limit-sticky9.c:841-893) maps a state s = (x, v) to
 //   Y1 = s + dt/2 F(s),  Y2 = s + dt/2 F(Y1),  s' = s + dt/6 (F(s) + F(Y1) + F(Y2))
@@ -107,10 +109,12 @@ int adj_get(pn1_ctx* c, pn1_adj** out)
 
 int nblk_of(const pn1_ctx* c) { return c->n > 1 ? (c->n - 1 + WG - 1) / WG : 1; }
 
-// j chunk of the single-pass pair kernel for nst stars in nIb blocks of
-// 256: a multiple of 64 records, as long as still gives about
+// j chunk L of the single-pass pair kernel for nst stars in nIb blocks of
+// 256: whole 64-record tiles, the longest that still gives about
 // PN1_ADJ_BLOCKS workgroups (several rounds of the whole device), within
-// [128, 2048].  The partials are (nst / L) * 7 + nIb * 6 rows of cap.
+// [128, 2048].  At n = 16385, L = 128 ... 1024 time within 2% of each
+// other.  The partials take ceil(nst / L) * 7 + nIb * 6 rows of cap doubles:
+// the j-side rows (one per 256-star block) dominate, ~n^2 / 5.3 bytes.
 #ifndef PN1_ADJ_BLOCKS
 #define PN1_ADJ_BLOCKS 2048
 #endif
