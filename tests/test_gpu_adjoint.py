@@ -89,6 +89,32 @@ def test_force_vjp_mode_independent(ctx):
         assert np.array_equal(np.asarray(a), np.asarray(b))
 
 
+def test_force_vjp_chunking_invariance(ctx):
+    """The single-pass kernel's j chunks (automatic, or set by an explicit j
+    split: 1, 3, 7, 64 chunks of whole 64-record tiles, ragged last tiles)
+    only reorder the sums: every chunking agrees to rounding, and each is
+    bitwise reproducible."""
+    x, v, m = pn.ic_plummer(1000, 16, a=1.0, m_bh=0.02)
+    u = np.random.default_rng(17).normal(size=x.shape)
+    ctx.set_params(1e-3, 50.0, 2.0**-10, pn.FP64)
+    ctx.upload(x, v, m)
+    out = {}
+    for split in (0, 1, 3, 7, 64):
+        ctx.set_tuning(split, 0, -1)
+        a = ctx.force_vjp(u)
+        b = ctx.force_vjp(u)
+        for p, q in zip(a, b):
+            assert np.array_equal(np.asarray(p), np.asarray(q)), split
+        out[split] = a
+    ctx.set_tuning(0, 0, -1)
+    ref = out[0]
+    scale = max(np.abs(ref[0]).max(), np.abs(ref[1]).max(), abs(ref[2]))
+    for split, (gx, gv, g0) in out.items():
+        err = max(np.abs(gx - ref[0]).max(), np.abs(gv - ref[1]).max(), abs(g0 - ref[2]))
+        print(f"jsplit {split}: max diff / scale {err / scale:.2e}")
+        assert err <= 1e-13 * scale, split
+
+
 def run_adjoint(ctx, x, v, m, eps, c, dt, steps, interval, r_cut):
     ctx.set_params(eps, c, dt, pn.FP64)
     ctx.upload(x, v, m)
