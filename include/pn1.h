@@ -148,8 +148,10 @@ int  pn1_force_vjp(pn1_ctx *ctx, const double *u, double *gx, double *gv,
  * pn1_adjoint_backward takes that gradient (gx, gv: d loss / d final x, v,
  * AoS [n][3]) and returns d loss / d initial x, v (gx0, gv0) and d loss /
  * d m[0] through the dynamics (*gm0, may be NULL).  It recomputes each
- * segment from its checkpoint and leaves the initial state resident.  It
- * fails with PN1_EINVAL if the state or eps, c, dt changed in between. */
+ * segment from its checkpoint, keeping the segment's stage states (device
+ * memory: ceil(steps / interval) + 3 interval states of 6 x capacity
+ * doubles), and leaves the initial state resident.  It fails with
+ * PN1_EINVAL if the state or eps, c, dt changed in between. */
 int  pn1_adjoint_forward(pn1_ctx *ctx, int steps, int interval);
 int  pn1_adjoint_backward(pn1_ctx *ctx, const double *gx, const double *gv,
                           double *gx0, double *gv0, double *gm0);

@@ -19,8 +19,9 @@
 //
 // pn1_adjoint_forward steps the state with the FP64 kernels and keeps a
 // checkpoint every `interval` steps; pn1_adjoint_backward re-runs each
-// segment from its checkpoint (keeping its states), then walks it backward,
-// recomputing Y1 and Y2 of each step with two force evaluations.
+// segment from its checkpoint, keeping every step's s, Y1 and Y2 (the
+// stage buffers H1, H2 of the step), then walks it backward with three
+// VJPs per step and no further force evaluation.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,7 +44,7 @@ struct pn1_adj {
     double* rpart = nullptr;    // pair kernel partials, rpart_n doubles
     size_t rpart_n = 0;
     double* ckpt = nullptr;     // nck states of 6 x cap
-    double* seg = nullptr;      // interval states of 6 x cap
+    double* seg = nullptr;      // per step of a segment: s, Y1, Y2 (6 x cap each)
     int nck_cap = 0, seg_cap = 0;
     // recorded by pn1_adjoint_forward
     bool have_fwd = false;
@@ -260,7 +261,7 @@ int pn1_adjoint_forward(pn1_ctx* c, int steps, int interval)
         if (a->seg) (void)hipFree(a->seg);
         a->seg = nullptr;
         a->seg_cap = 0;
-        if ((rc = aalloc(c, &a->seg, (size_t)seg * s6))) return rc;
+        if ((rc = aalloc(c, &a->seg, 3 * (size_t)seg * s6))) return rc;
         a->seg_cap = seg;
     }
     Fp64Scope fp64(c);
@@ -299,24 +300,28 @@ int pn1_adjoint_backward(pn1_ctx* c, const double* gx, const double* gv, double*
     const int nck = (a->steps + a->interval - 1) / a->interval;
     for (int k = nck - 1; k >= 0; --k) {
         const int t0 = k * a->interval, t1 = std::min(a->steps, t0 + a->interval);
-        // states of the segment
+        // the segment's states: per step s, Y1 = H1 and Y2 = H2 (the last
+        // step runs two stages only: its end state is not needed)
         if ((rc = copy6(c, c->X, a->ckpt + (size_t)k * s6))) return rc;
         for (int t = t0; t < t1; ++t) {
-            if ((rc = copy6(c, a->seg + (size_t)(t - t0) * s6, c->X))) return rc;
-            if (t + 1 < t1 && (rc = enqueue_stages(c, 3, P))) return rc;
+            double* st = a->seg + 3 * (size_t)(t - t0) * s6;
+            if ((rc = copy6(c, st, c->X))) return rc;
+            if ((rc = enqueue_stages(c, t + 1 < t1 ? 3 : 2, P))) return rc;
+            if ((rc = copy6(c, st + s6, c->H1))) return rc;
+            if ((rc = copy6(c, st + 2 * s6, c->H2))) return rc;
         }
         for (int t = t1 - 1; t >= t0; --t) {
-            if ((rc = copy6(c, c->X, a->seg + (size_t)(t - t0) * s6))) return rc;
-            if ((rc = enqueue_stages(c, 2, P))) return rc;     // H1 = Y1, H2 = Y2
+            const double* st = a->seg + 3 * (size_t)(t - t0) * s6;
             lin(c, a->mu, dt / 6.0, a->L);
-            if ((rc = enqueue_vjp(c, a, c->H2, a->mu, a->LY2, P))) return rc;
+            if ((rc = enqueue_vjp(c, a, st + 2 * s6, a->mu, a->LY2, P))) return rc;
             lin(c, a->mu, dt / 6.0, a->L, dt / 2.0, a->LY2);
-            if ((rc = enqueue_vjp(c, a, c->H1, a->mu, a->LY1, P))) return rc;
+            if ((rc = enqueue_vjp(c, a, st + s6, a->mu, a->LY1, P))) return rc;
             lin(c, a->mu, dt / 6.0, a->L, dt / 2.0, a->LY1);
-            if ((rc = enqueue_vjp(c, a, c->X, a->mu, a->T, P))) return rc;
+            if ((rc = enqueue_vjp(c, a, st, a->mu, a->T, P))) return rc;
             lin(c, a->L, 1.0, a->L, 1.0, a->LY2, 1.0, a->LY1, 1.0, a->T);
         }
     }
+    if ((rc = copy6(c, c->X, a->ckpt))) return rc;          // leave the initial state
     HIPCHK(c, hipGetLastError());
     const size_t cap = (size_t)c->cap;
     if ((rc = fetch3(c, a->L, gx0))) return rc;
@@ -488,7 +493,7 @@ struct pn1_ens_adj {
     double *T = nullptr, *ablk = nullptr, *gm0 = nullptr;
     ARec* rec = nullptr;
     double* ckpt = nullptr;
-    double* seg = nullptr;
+    double* seg = nullptr;        // per step of a segment: s, Y1, Y2 (6 x cap each)
     int nck_cap = 0, seg_cap = 0, ablk_cap = 0;
     int grid = 0;                 // resident workgroups of k_ens_adj_rev
     double* apart = nullptr;      // pair kernel partials, apart_n doubles
@@ -725,7 +730,7 @@ int pn1_ens_adjoint_forward(pn1_ens* e, int steps, int interval)
         if (a->seg) (void)hipFree(a->seg);
         a->seg = nullptr;
         a->seg_cap = 0;
-        if ((rc = ealloc_adj(e, &a->seg, (size_t)seg * s6))) return rc;
+        if ((rc = ealloc_adj(e, &a->seg, 3 * (size_t)seg * s6))) return rc;
         a->seg_cap = seg;
     }
     EnsFp64Scope fp64(e);
@@ -768,21 +773,24 @@ int pn1_ens_adjoint_backward(pn1_ens* e, const double* gx, const double* gv, dou
         const int t0 = k * a->interval, t1 = std::min(a->steps, t0 + a->interval);
         if ((rc = ens_copy6(e, e->X, a->ckpt + (size_t)k * s6))) return rc;
         for (int t = t0; t < t1; ++t) {
-            if ((rc = ens_copy6(e, a->seg + (size_t)(t - t0) * s6, e->X))) return rc;
-            if (t + 1 < t1 && (rc = ens_stages(e, 3, P))) return rc;
+            double* st = a->seg + 3 * (size_t)(t - t0) * s6;
+            if ((rc = ens_copy6(e, st, e->X))) return rc;
+            if ((rc = ens_stages(e, t + 1 < t1 ? 3 : 2, P))) return rc;
+            if ((rc = ens_copy6(e, st + s6, e->H1))) return rc;
+            if ((rc = ens_copy6(e, st + 2 * s6, e->H2))) return rc;
         }
         for (int t = t1 - 1; t >= t0; --t) {
-            if ((rc = ens_copy6(e, e->X, a->seg + (size_t)(t - t0) * s6))) return rc;
-            if ((rc = ens_stages(e, 2, P))) return rc;
+            const double* st = a->seg + 3 * (size_t)(t - t0) * s6;
             ens_lin(e, a->mu, dt / 6.0, a->L);
-            if ((rc = ens_enqueue_vjp(e, a, e->H2, a->mu, a->LY2, P))) return rc;
+            if ((rc = ens_enqueue_vjp(e, a, st + 2 * s6, a->mu, a->LY2, P))) return rc;
             ens_lin(e, a->mu, dt / 6.0, a->L, dt / 2.0, a->LY2);
-            if ((rc = ens_enqueue_vjp(e, a, e->H1, a->mu, a->LY1, P))) return rc;
+            if ((rc = ens_enqueue_vjp(e, a, st + s6, a->mu, a->LY1, P))) return rc;
             ens_lin(e, a->mu, dt / 6.0, a->L, dt / 2.0, a->LY1);
-            if ((rc = ens_enqueue_vjp(e, a, e->X, a->mu, a->T, P))) return rc;
+            if ((rc = ens_enqueue_vjp(e, a, st, a->mu, a->T, P))) return rc;
             ens_lin(e, a->L, 1.0, a->L, 1.0, a->LY2, 1.0, a->LY1, 1.0, a->T);
         }
     }
+    if ((rc = ens_copy6(e, e->X, a->ckpt))) return rc;      // leave the initial state
     EHIPCHK(e, hipGetLastError());
     const size_t cap = (size_t)e->cap;
     if ((rc = ens_fetch3(e, a->L, gx0))) return rc;
