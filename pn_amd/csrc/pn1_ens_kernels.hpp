@@ -51,7 +51,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
                                                   double* __restrict__ part,
                                                   const EItem* __restrict__ items,
                                                   unsigned int* __restrict__ counter,
-                                                  const int* __restrict__ qoff, Phys P, EConst E)
+                                                  const int* __restrict__ qoff, Phys P)
 {
     int q = 0;
     for (;;) {
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
         if (it < 0) break;
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
-        const I64 I = load_i64(jr, st, P.cap, g.i, 0.0, P.eps2);
+        const I64 I = load_i64(jr, st, P.cap, g.i, P.eps2);
         double a[NACC];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0;
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
                                                   float* __restrict__ part,
                                                   const EItem* __restrict__ items,
                                                   unsigned int* __restrict__ counter,
-                                                  const int* __restrict__ qoff, Phys P, EConst E,
+                                                  const int* __restrict__ qoff, Phys P,
                                                   int tile)
 {
     int q = 0;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
         if (it < 0) break;
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
-        const I32 I = load_i32(jr, st, P.cap, g.i, 0.0, P.eps2);
+        const I32 I = load_i32(jr, st, P.cap, g.i, P.eps2);
         float a[NACC], sN[3], cN[3];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
@@ -218,8 +218,7 @@ __global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ j
                                                    double* __restrict__ part,
                                                    const EItem* __restrict__ items,
                                                    unsigned int* __restrict__ counter,
-                                                   const int* __restrict__ qoff, Phys P,
-                                                   EConst E)
+                                                   const int* __restrict__ qoff, Phys P)
 {
     int q = 0;
     for (;;) {

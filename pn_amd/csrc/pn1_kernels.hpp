@@ -147,7 +147,7 @@ struct I64 {
 
 __device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr,
                                         const double* __restrict__ st, int cap, int i,
-                                        double m0, double eps2)
+                                        double eps2)
 {
     const JRec64& r = jr[i];
     const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I64 I = load_i64(jr, st, P.cap, i, P.m0, P.eps2);
+    const I64 I = load_i64(jr, st, P.cap, i, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const double eps2 = P.eps2;
     double a[NACC];
@@ -271,7 +271,7 @@ struct I32 {
 
 __device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
                                         const double* __restrict__ st, int cap, int i,
-                                        double m0, double eps2)
+                                        double eps2)
 {
     const JRec32& r = jr[i];
     const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
@@ -414,7 +414,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(jr, st, P.cap, i, P.m0, P.eps2);
+    const I32 I = load_i32(jr, st, P.cap, i, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
     float a[NACC], sumN[3], cmpN[3];
