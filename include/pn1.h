@@ -110,10 +110,11 @@ const char *pn1_last_error(const pn1_ctx *ctx);
  * rsqrt.  pn1_get_tuning reports the values in use.  Used by the tuner. */
 int  pn1_set_tuning(pn1_ctx *ctx, int jsplit, int tile, int variant);
 int  pn1_get_tuning(const pn1_ctx *ctx, int *jsplit, int *tile, int *variant);
-/* Autotuner: times the pair kernel of `reps` force evaluations of the
- * resident state for each j split (automatic, 8 ... 256) and j loop
- * variant, keeps the fastest (as pn1_set_tuning would) and reports it
- * (any output may be NULL).  The mixed form's compensation interval is
+/* Autotuner: times `reps` whole force evaluations of the resident state
+ * (every kernel, so the cost of reducing the split's partial sums counts)
+ * for each j split (automatic, 8 ... 256) and j loop variant, keeps the
+ * fastest (as pn1_set_tuning would) and reports it with its time per
+ * evaluation in ms (any output may be NULL).  The mixed form's compensation interval is
  * not changed: it trades accuracy.  No counterpart in the reference. */
 int  pn1_autotune(pn1_ctx *ctx, int reps, int *jsplit, int *variant,
                   double *best_ms);

@@ -150,6 +150,7 @@ bool ready(pn1_ctx* c, int* rc)
         *rc = seterr(c, PN1_EDEVICE, "hipSetDevice(%d) failed", c->dev);
         return false;
     }
+    *rc = PN1_OK;
     return true;
 }
 
@@ -543,20 +544,32 @@ int pn1_autotune(pn1_ctx* c, int reps, int* jsplit, int* variant, double* best_m
     const Phys P = make_phys(c);
     double best = 1e300;
     int bS = 0, bV = -1;
+    // whole force evaluations (every kernel: the partial-sum reduction in
+    // k_fin grows with the split), timed with events on the stream
+    hipEvent_t ea = nullptr, eb = nullptr;
+    if (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess) {
+        if (ea) (void)hipEventDestroy(ea);
+        return seterr(c, PN1_EDEVICE, "hipEventCreate failed");
+    }
+    if ((rc = pn1_profile_enable(c, 0))) {
+        (void)hipEventDestroy(ea);
+        (void)hipEventDestroy(eb);
+        return rc;
+    }
     auto time_shape = [&](int S, int var, double* avg) {
         c->jsplit_req = S;
         c->variant_req = var;
         c->tile_req = keep_tile;
         plan(c);
         int r = ensure_part(c);
-        if (!r) r = pn1_profile_enable(c, 1);
         if (!r) r = enqueue_force(c, 0, P);                      // warm-up
-        if (!r) r = pn1_profile_enable(c, 1);
+        if (!r && hipEventRecord(ea, c->st) != hipSuccess) r = seterr(c, PN1_EDEVICE, "hipEventRecord failed");
         for (int k = 0; k < reps && !r; ++k) r = enqueue_force(c, 0, P);
-        int k = 0;
-        double ms = 0.0;
-        if (!r) r = pn1_profile_read(c, &k, &ms);
-        *avg = k ? ms / k : 0.0;
+        if (!r && hipEventRecord(eb, c->st) != hipSuccess) r = seterr(c, PN1_EDEVICE, "hipEventRecord failed");
+        float ms = 0.0f;
+        if (!r && (hipEventSynchronize(eb) != hipSuccess || hipEventElapsedTime(&ms, ea, eb) != hipSuccess))
+            r = seterr(c, PN1_EDEVICE, "timing the force evaluations failed");
+        *avg = (double)ms / reps;
         return r;
     };
     for (int S : splits) {
@@ -571,6 +584,8 @@ int pn1_autotune(pn1_ctx* c, int reps, int* jsplit, int* variant, double* best_m
         }
         if (rc) break;
     }
+    (void)hipEventDestroy(ea);
+    (void)hipEventDestroy(eb);
     (void)pn1_profile_enable(c, was_prof ? 1 : 0);
     if (rc) {                              // leave the shape the caller had
         c->jsplit_req = keep_S;

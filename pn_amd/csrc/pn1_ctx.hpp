@@ -51,7 +51,8 @@ pn1dev::Phys make_phys(const pn1_ctx* c);
 int enqueue_force(pn1_ctx* c, int stage, const pn1dev::Phys& P);
 // SoA device [k][cap] -> AoS host [n][3] for k in 0..2, through the stage.
 int fetch3(pn1_ctx* c, const double* dsrc, double* out);
-// Checks that a state is resident and makes the context's device current.
+// Checks that a state is resident and makes the context's device current;
+// *rc is PN1_OK, or the error when it returns false.
 bool ready(pn1_ctx* c, int* rc);
 void drop_graph(pn1_ctx* c);
 void adj_free(pn1_ctx* c);   // pn1_adj.hip
