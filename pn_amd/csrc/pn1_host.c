@@ -172,7 +172,9 @@ int pn1_ic_uniform_sphere(int n_stars, unsigned long long seed, double R,
         const double *vi = v + 3 * i;
         K += 0.5 * mi * (vi[0] * vi[0] + vi[1] * vi[1] + vi[2] * vi[2]);
     }
-    const double scale = sqrt(Q * fabs(W) / K);
+    /* K = 0 only for a single star, whose velocity is zero once the mean is
+     * removed (and which then sits on the BH: pn1_check_input rejects it) */
+    const double scale = K > 0.0 ? sqrt(Q * fabs(W) / K) : 0.0;
     for (int i = 1; i <= n_stars; i++)
         for (int k = 0; k < 3; k++) v[3 * i + k] = f32(v[3 * i + k] * scale);
     return PN1_OK;
