@@ -330,6 +330,30 @@ def run_adjoint(steps, interval, vjp_reps=5):
     }
 
 
+def run_shard(ctx, x, v, m, shards=8, reps=5):
+    """One GPU's share of a run whose stars are split across `shards` GPUs
+    (every particle's state all-gathered first): the pair kernel over the
+    first n/shards stars, in whole 256-star blocks, against all n particles,
+    timed alone on this GPU through pn1_force_range (HIP events around the
+    pair launch)."""
+    n = len(m)
+    i1 = 1 + ((n - 1) // shards + 255) // 256 * 256
+    out = {"shards": shards, "i_stars": i1 - 1, "j_particles": n}
+    for mode, name in ((pn.MIXED, "mixed"), (pn.FP64, "fp64")):
+        ctx.set_params(EPS, C_LIGHT, DT, mode)
+        ctx.upload(x, v, m)
+        ctx.force_range(1, i1)                      # warm-up
+        ctx.profile_enable(True)
+        for _ in range(reps):
+            ctx.force_range(1, i1)
+        k, ms = ctx.profile_read()
+        ctx.profile_enable(False)
+        pair_ms = ms / max(k, 1)
+        out[name] = {"pair_ms": pair_ms,
+                     "interactions_per_s": (i1 - 1) * n / (pair_ms * 1e-3)}
+    return out
+
+
 def run_drift(x, v, m, steps, every=100, track=10):
     """Relative energy drift (E_t - E_0) / |E_0| of both forms over `steps`
     steps of the headline workload, sampled every `every` steps, and an
@@ -443,6 +467,9 @@ def main():
         fp64 = run_form(ctx, x, v, m, pn.FP64, args.steps, args.warmup, tune)
         mixed = run_form(ctx, x, v, m, pn.MIXED, args.steps, args.warmup, tune)
         jsplit, tile, variant = mixed["shape"]
+        shard = run_shard(ctx, x, v, m)
+        for name, form in (("mixed", mixed), ("fp64", fp64)):
+            shard[name]["fraction_of_full_launch"] = shard[name]["pair_ms"] / form["pair_avg_ms"]
 
     out = {
         "metric": METRIC,
@@ -482,6 +509,8 @@ def main():
                                    "energy_drift_mixed": mixed["energy_drift"]},
         "pair_kernel_avg_ms": {"mixed": mixed["pair_avg_ms"], "fp64": fp64["pair_avg_ms"],
                                "launches": mixed["pair_launches"]},
+        # one GPU's i shard of an 8-GPU run at this size (stars split, j all-gathered)
+        "shard_of_8": shard,
         "flops_per_interaction": FLOPS_PER_INTERACTION,
         # the literature's Newtonian-only convention (20 flops per interaction)
         "tflops_newtonian_convention": {"mixed": 20.0 * mixed["interactions_per_s"] / 1e12,

@@ -91,6 +91,15 @@ int  pn1_upload(pn1_ctx *ctx, const double *x, const double *v,
 int  pn1_download(pn1_ctx *ctx, double *x, double *v);
 /* One force evaluation on the resident state; a is AoS [n][3]. */
 int  pn1_force(pn1_ctx *ctx, double *a);
+/* The accelerations of stars i0 .. i1 - 1 only, against every particle:
+ * the pair kernel runs over those stars' 256-star blocks alone.  This is
+ * one rank's work in a run whose stars are split across GPUs (each rank
+ * holding every particle's state after an all-gather); the BH, which sums
+ * over every star, is not evaluated.  i0 - 1 and i1 - 1 must be multiples
+ * of 256 (or i1 = n); a is AoS [i1 - i0][3] and equals rows i0 .. i1 - 1 of
+ * pn1_force bit for bit.  No counterpart in the reference, which runs on
+ * one processor. */
+int  pn1_force_range(pn1_ctx *ctx, int i0, int i1, double *a);
 /* Energy of the resident state. */
 int  pn1_energy(pn1_ctx *ctx, pn1_energy_t *out);
 /* `steps` steps of the reference's three-evaluation stage scheme

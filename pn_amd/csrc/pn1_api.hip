@@ -159,9 +159,43 @@ template <int MODE, int STAGE>
 void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
 {
     hipLaunchKernelGGL((k_fin<MODE, STAGE>), dim3(c->nblk), dim3(WG), 0, c->st, in, c->X, c->Y,
-                       c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P);
+                       c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P,
+                       1, c->n);
     hipLaunchKernelGGL((k_bh<STAGE>), dim3(1), dim3(WG), 0, c->st, c->bhblk, c->nblk, c->gs, c->X,
                        c->Y, c->H1, c->H2, c->acc, P);
+}
+
+// The pair kernel over i blocks ib0 .. ib0 + nib - 1 against every star.
+void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib)
+{
+    const bool mixed = c->mode == PN1_MIXED;
+    const int S = c->S;
+    int grid = S * nib;
+    if (S <= 8 && 8 % S == 0) {
+        const int per = 8 / S;
+        grid = 8 * ((nib + per - 1) / per);
+    }
+    const dim3 g(grid), blk(WG);
+    const int variant = kernel_variant(c);
+    if (mixed) {
+        float* pp = (float*)c->part;
+        switch (variant) {
+        case 0: hipLaunchKernelGGL(k_pair32<0>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+                                   c->tile); break;
+        case 1: hipLaunchKernelGGL(k_pair32<1>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+                                   c->tile); break;
+        case 2: hipLaunchKernelGGL(k_pair32<2>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+                                   c->tile); break;
+        default: hipLaunchKernelGGL(k_pair32<3>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+                                    c->tile); break;
+        }
+    } else if (variant & 1) {
+        hipLaunchKernelGGL(k_pair64<1>, g, blk, 0, c->st, c->j64, in, (double*)c->part, P, S, nib,
+                           ib0);
+    } else {
+        hipLaunchKernelGGL(k_pair64<0>, g, blk, 0, c->st, c->j64, in, (double*)c->part, P, S, nib,
+                           ib0);
+    }
 }
 
 // stage 0: plain evaluation on X; stages 1..3: the driver's force calls.
@@ -179,28 +213,7 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
             HIPCHK(c, hipEventCreate(&e1));
             HIPCHK(c, hipEventRecord(e0, c->st));
         }
-        const dim3 grid(c->pair_grid), blk(WG);
-        const int variant = kernel_variant(c);
-        if (mixed) {
-            float* pp = (float*)c->part;
-            switch (variant) {
-            case 0: hipLaunchKernelGGL(k_pair32<0>, grid, blk, 0, c->st, c->j32, in, pp, P, c->S,
-                                       c->nIblk, c->tile); break;
-            case 1: hipLaunchKernelGGL(k_pair32<1>, grid, blk, 0, c->st, c->j32, in, pp, P, c->S,
-                                       c->nIblk, c->tile); break;
-            case 2: hipLaunchKernelGGL(k_pair32<2>, grid, blk, 0, c->st, c->j32, in, pp, P, c->S,
-                                       c->nIblk, c->tile); break;
-            default: hipLaunchKernelGGL(k_pair32<3>, grid, blk, 0, c->st, c->j32, in, pp, P, c->S,
-                                        c->nIblk, c->tile); break;
-            }
-        } else {
-            if (variant & 1)
-                hipLaunchKernelGGL(k_pair64<1>, grid, blk, 0, c->st, c->j64, in,
-                                   (double*)c->part, P, c->S, c->nIblk);
-            else
-                hipLaunchKernelGGL(k_pair64<0>, grid, blk, 0, c->st, c->j64, in,
-                                   (double*)c->part, P, c->S, c->nIblk);
-        }
+        launch_pair(c, in, P, 0, c->nIblk);
         if (c->prof) {
             HIPCHK(c, hipEventRecord(e1, c->st));
             c->ev.emplace_back(e0, e1);
@@ -419,6 +432,53 @@ int pn1_force(pn1_ctx* c, double* a)
     if (!a) return seterr(c, PN1_EINVAL, "null output");
     if ((rc = enqueue_force(c, 0, make_phys(c)))) return rc;
     return fetch3(c, c->acc, a);
+}
+
+int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
+{
+    int rc;
+    if (!ready(c, &rc)) return rc;
+    if (!a) return seterr(c, PN1_EINVAL, "null output");
+    const int n = c->n;
+    if (i0 < 1 || i1 > n || i0 >= i1 || (i0 - 1) % WG != 0 || ((i1 - 1) % WG != 0 && i1 != n))
+        return seterr(c, PN1_EINVAL,
+                      "star range [%d, %d) must lie in [1, %d) and start and end on %d-star "
+                      "block boundaries (or end at n)", i0, i1, n, WG);
+    const Phys P = make_phys(c);
+    const bool mixed = c->mode == PN1_MIXED;
+    const double* in = c->X;
+    // every star's j-record and the separable sums: what a sharded run would
+    // all-gather and all-reduce before its pair kernel
+    hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
+                       mixed ? c->j32 : nullptr, c->gblk);
+    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof) {
+        HIPCHK(c, hipEventCreate(&e0));
+        HIPCHK(c, hipEventCreate(&e1));
+        HIPCHK(c, hipEventRecord(e0, c->st));
+    }
+    launch_pair(c, in, P, (i0 - 1) / WG, (i1 - i0 + WG - 1) / WG);
+    if (c->prof) {
+        HIPCHK(c, hipEventRecord(e1, c->st));
+        c->ev.emplace_back(e0, e1);
+    }
+    const dim3 gf((i1 - i0 + WG - 1) / WG);
+    if (mixed)
+        hipLaunchKernelGGL((k_fin<1, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
+                           c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P, i0, i1);
+    else
+        hipLaunchKernelGGL((k_fin<0, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
+                           c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P, i0, i1);
+    HIPCHK(c, hipGetLastError());
+    const size_t cap = (size_t)c->cap, cnt = (size_t)(i1 - i0);
+    for (int k = 0; k < 3; ++k)
+        HIPCHK(c, hipMemcpyAsync(c->h_stage + k * cap, c->acc + k * cap + i0,
+                                 cnt * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t i = 0; i < cnt; ++i)
+        for (int k = 0; k < 3; ++k) a[3 * i + k] = c->h_stage[k * cap + i];
+    return PN1_OK;
 }
 
 int pn1_energy(pn1_ctx* c, pn1_energy_t* out)

@@ -230,15 +230,18 @@ __device__ __forceinline__ void run64(const JRec64* __restrict__ jr, int j0, int
     if (j < j1) body64<MASK>(ra, j, i, I, a, eps2);
 }
 
+// i blocks ib0 .. ib0 + nIblk - 1 (ib0 = 0: every star; a later first block
+// evaluates one i shard of the stars against all of them).
 template <int PP>
 __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
                                               const double* __restrict__ st,
                                               double* __restrict__ part, Phys P, int S,
-                                              int nIblk)
+                                              int nIblk, int ib0)
 {
     int s, ib;
     map_block(blockIdx.x, S, s, ib);
     if (ib >= nIblk) return;
+    ib += ib0;
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
@@ -406,11 +409,12 @@ template <int PP>
 __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
                                               const double* __restrict__ st,
                                               float* __restrict__ part, Phys P, int S,
-                                              int nIblk, int tile)
+                                              int nIblk, int ib0, int tile)
 {
     int s, ib;
     map_block(blockIdx.x, S, s, ib);
     if (ib >= nIblk) return;
+    ib += ib0;
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
@@ -568,17 +572,19 @@ __device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
     return C;
 }
 
+// Stars p0 .. p1 - 1 (p0 = 1, p1 = n: every star; a sub-range is one i shard
+// of a plain evaluation, STAGE 0).
 template <int MODE, int STAGE>
 __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double* Y, double* H1,
                                            double* H2, const double* __restrict__ m,
                                            const void* __restrict__ part, int S,
                                            const double* __restrict__ gs,
                                            double* __restrict__ acc,
-                                           double* __restrict__ bhblk, Phys P)
+                                           double* __restrict__ bhblk, Phys P, int p0, int p1)
 {
-    const int p = blockIdx.x * WG + threadIdx.x + 1;
+    const int p = p0 + blockIdx.x * WG + threadIdx.x;
     double cb[3] = {0.0, 0.0, 0.0};
-    if (p < P.n) {
+    if (p < p1) {
         double S_[NACC];
         gather_acc<MODE>(part, S, (size_t)P.cap, p, S_);
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb);

@@ -59,6 +59,7 @@ def load():
         "pn1_upload": (_I, [P, _D, _D, _D, _I]),
         "pn1_download": (_I, [P, _D, _D]),
         "pn1_force": (_I, [P, _D]),
+        "pn1_force_range": (_I, [P, _I, _I, _D]),
         "pn1_energy": (_I, [P, ctypes.POINTER(Energy)]),
         "pn1_step": (_I, [P, _I]),
         "pn1_sync": (_I, [P]),
@@ -193,6 +194,13 @@ class Context:
     def force(self):
         a = np.empty((self.n, 3))
         self._chk(self._L.pn1_force(self._h, ptr(a)), "pn1_force")
+        return a
+
+    def force_range(self, i0, i1):
+        """Accelerations of stars i0 .. i1 - 1 only (one i shard; 256-star
+        block boundaries), [i1 - i0, 3]."""
+        a = np.empty((max(int(i1) - int(i0), 0), 3))
+        self._chk(self._L.pn1_force_range(self._h, int(i0), int(i1), ptr(a)), "pn1_force_range")
         return a
 
     def energy(self):
