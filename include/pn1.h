@@ -96,9 +96,11 @@ int  pn1_force(pn1_ctx *ctx, double *a);
  * one rank's work in a run whose stars are split across GPUs (each rank
  * holding every particle's state after an all-gather); the BH, which sums
  * over every star, is not evaluated.  i0 - 1 and i1 - 1 must be multiples
- * of 256 (or i1 = n); a is AoS [i1 - i0][3] and equals rows i0 .. i1 - 1 of
- * pn1_force bit for bit.  No counterpart in the reference, which runs on
- * one processor. */
+ * of 256 (or i1 = n); a is AoS [i1 - i0][3].  With an explicit j split
+ * (pn1_set_tuning, pn1_autotune) it equals rows i0 .. i1 - 1 of pn1_force
+ * bit for bit; otherwise the shard gets a split of its own (more chunks
+ * for its smaller grid) and agrees to rounding.  No counterpart in the
+ * reference, which runs on one processor. */
 int  pn1_force_range(pn1_ctx *ctx, int i0, int i1, double *a);
 /* Energy of the resident state. */
 int  pn1_energy(pn1_ctx *ctx, pn1_energy_t *out);

@@ -102,15 +102,17 @@ void plan(pn1_ctx* c)
     c->tile = c->tile_req > 0 ? (c->tile_req < 2 ? 2 : c->tile_req) : 16;
 }
 
-int ensure_part(pn1_ctx* c)
+// Partial-sum buffer for a j split of S chunks (default: the plan's).
+int ensure_part(pn1_ctx* c, int split = 0)
 {
-    const size_t cap = (size_t)c->cap, S = (size_t)c->S;
+    const size_t cap = (size_t)c->cap, S = (size_t)(split > 0 ? split : c->S);
     size_t need = S * NACC * cap * sizeof(double);
     const size_t needE = S * NACCE * cap * sizeof(double);
     const size_t need32 = S * (NACC + 3) * cap * sizeof(float);
     if (needE > need) need = needE;
     if (need32 > need) need = need32;
     if (need <= c->part_bytes) return PN1_OK;
+    drop_graph(c);                      // a captured step holds the old buffer
     if (c->part) (void)hipFree(c->part);
     c->part = nullptr;
     c->part_bytes = 0;
@@ -165,11 +167,11 @@ void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
                        c->Y, c->H1, c->H2, c->acc, P);
 }
 
-// The pair kernel over i blocks ib0 .. ib0 + nib - 1 against every star.
-void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib)
+// The pair kernel over i blocks ib0 .. ib0 + nib - 1 against every star,
+// with the j range split in S chunks.
+void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S)
 {
     const bool mixed = c->mode == PN1_MIXED;
-    const int S = c->S;
     int grid = S * nib;
     if (S <= 8 && 8 % S == 0) {
         const int per = 8 / S;
@@ -213,7 +215,7 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
             HIPCHK(c, hipEventCreate(&e1));
             HIPCHK(c, hipEventRecord(e0, c->st));
         }
-        launch_pair(c, in, P, 0, c->nIblk);
+        launch_pair(c, in, P, 0, c->nIblk, c->S);
         if (c->prof) {
             HIPCHK(c, hipEventRecord(e1, c->st));
             c->ev.emplace_back(e0, e1);
@@ -447,6 +449,18 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     const Phys P = make_phys(c);
     const bool mixed = c->mode == PN1_MIXED;
     const double* in = c->X;
+    const int nib = (i1 - i0 + WG - 1) / WG;
+    // The j split: an explicit one (pn1_set_tuning, pn1_autotune) as set,
+    // which keeps the rows bitwise equal to pn1_force's; otherwise the
+    // shard's own plan, the full launch's split doubled while the shard's
+    // grid has fewer than 16384 workgroups (chunks of >= 64 records, at most
+    // 512 chunks).
+    int S = c->S;
+    if (c->jsplit_req <= 0) {
+        const int nst = n - 1;
+        while (S < 512 && (long)S * nib < 16384 && (nst + 2 * S - 1) / (2 * S) >= 64) S *= 2;
+    }
+    if ((rc = ensure_part(c, S))) return rc;
     // every star's j-record and the separable sums: what a sharded run would
     // all-gather and all-reduce before its pair kernel
     hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
@@ -458,18 +472,18 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
         HIPCHK(c, hipEventCreate(&e1));
         HIPCHK(c, hipEventRecord(e0, c->st));
     }
-    launch_pair(c, in, P, (i0 - 1) / WG, (i1 - i0 + WG - 1) / WG);
+    launch_pair(c, in, P, (i0 - 1) / WG, nib, S);
     if (c->prof) {
         HIPCHK(c, hipEventRecord(e1, c->st));
         c->ev.emplace_back(e0, e1);
     }
-    const dim3 gf((i1 - i0 + WG - 1) / WG);
+    const dim3 gf(nib);
     if (mixed)
         hipLaunchKernelGGL((k_fin<1, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P, i0, i1);
+                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, P, i0, i1);
     else
         hipLaunchKernelGGL((k_fin<0, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P, i0, i1);
+                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, P, i0, i1);
     HIPCHK(c, hipGetLastError());
     const size_t cap = (size_t)c->cap, cnt = (size_t)(i1 - i0);
     for (int k = 0; k < 3; ++k)
