@@ -378,6 +378,7 @@ __global__ __launch_bounds__(WG) void k_ens_adj_rev(const ARec* __restrict__ rec
                                                    const int* __restrict__ qoff, Phys P, int L)
 {
     __shared__ double sh[NADJ2][4][64];
+    __shared__ double tls[4][REVJ_F][64];
     const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     int q = 0;
     for (;;) {
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(WG) void k_ens_adj_rev(const ARec* __restrict__ rec
         double* pj = base + (size_t)S * NADJ1 * nst;
         for (int jt = w.j0; jt < w.j1; jt += 64) {
             double g[NADJ2];
-            rev_tile(rec, I, U, i, iok, jt, w.j1, ai, g);
+            rev_tile(rec, I, U, i, iok, jt, w.j1, ai, g, tls[wv]);
 #pragma unroll
             for (int k = 0; k < NADJ2; ++k) sh[k][wv][lane] = g[k];
             __syncthreads();

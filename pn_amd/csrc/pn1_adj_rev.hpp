@@ -287,18 +287,21 @@ __host__ __device__ __forceinline__ void rev_jfin(const RevJ& J, const RevU& U, 
 // Device: the single-pass pair kernel.
 //
 // A workgroup (4 waves) takes 256 stars i (one per lane, fixed) and a
-// chunk of j records, in tiles of 64.  Within a tile, step t pairs lane l
-// with record jt + ((l + t) mod 64): the j record and its running sums move
-// one lane down after every step (a DPP lane rotation), so each lane meets
-// every record of the tile once and, after 64 steps, every
-// record is back on its home lane with its sum over the wave's 64 stars.
-// The four waves' sums are added in wave order through LDS and written
-// per (i block, j); the i side is written per (j chunk, i).  Every order
-// is fixed, so the result is bitwise reproducible.
+// chunk of j records, in tiles of 64.  Each wave stages the tile's records
+// in LDS (read-only for the tile).  Step t pairs lane l with record
+// jt + ((l + t) mod 64), read from LDS, while that record's running j-side
+// sums move one lane down after every step (a DPP lane rotation).  So each
+// lane meets every record of the tile once and, after 64 steps, every
+// record's sums are back on its home lane, summed over the wave's 64 stars.
+// The four waves' sums are added in wave order through LDS and written per
+// (i block, j); the i side is written per (j chunk, i).  Every order is
+// fixed, so the result is bitwise reproducible.
 // =====================================================================
 // The value v of lane (lane + 1) mod 64: DPP wave_rol:1 on each half
 // (v_mov_b32_dpp, a VALU move with no LDS round trip, so nothing waits on
-// it; measured 12% faster for this kernel than ds_bpermute).
+// it; measured 12% faster for this kernel than ds_bpermute).  The records
+// themselves are read from LDS instead (another 3%: their 18 moves per step
+// leave the VALU).
 __device__ __forceinline__ double lane_next(double v)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -306,20 +309,6 @@ __device__ __forceinline__ double lane_next(double v)
     const unsigned hi =
         (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x134, 0xf, 0xf, true);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-__device__ __forceinline__ RevJ rev_next(const RevJ& J)
-{
-    RevJ R;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        R.x[k] = lane_next(J.x[k]);
-        R.v[k] = lane_next(J.v[k]);
-    }
-    R.rc2 = lane_next(J.rc2);
-    R.rce = lane_next(J.rce);
-    R.m = lane_next(J.m);
-    return R;
 }
 
 __device__ __forceinline__ void rev_next(RevJAcc& a)
@@ -335,26 +324,40 @@ __device__ __forceinline__ void rev_next(RevJAcc& a)
 
 // One tile of records [jt, min(jt + 64, jend)) against the lane's star i
 // (masked when !iok, and for j = i).  The i side accumulates into ai; g
-// gets the tile's j-side sums of the lane's home record jt + lane.
+// gets the tile's j-side sums of the lane's home record jt + lane.  tl is
+// the wave's LDS tile, field-major (at step t the lanes read one field of
+// records (l + t) mod 64, consecutive doubles: no bank conflicts).  Called
+// by every thread of the workgroup (it holds a barrier); the caller's
+// barriers after the tile keep the next tile's stores behind these reads.
+constexpr int REVJ_F = 9;
 __device__ __forceinline__ void rev_tile(const ARec* __restrict__ rec, const RevI& I,
                                          const RevU& U, int i, bool iok, int jt, int jend,
-                                         RevIAcc& ai, double (&g)[NADJ2])
+                                         RevIAcc& ai, double (&g)[NADJ2],
+                                         double (*tl)[64])
 {
     const int lane = (int)(threadIdx.x & 63);
     const int jl = jt + lane;
-    RevJ J = rev_jrec(rec[jl < jend ? jl : jt]);
+    const RevJ J0 = rev_jrec(rec[jl < jend ? jl : jt]);
+    tl[0][lane] = J0.x[0]; tl[1][lane] = J0.x[1]; tl[2][lane] = J0.x[2];
+    tl[3][lane] = J0.v[0]; tl[4][lane] = J0.v[1]; tl[5][lane] = J0.v[2];
+    tl[6][lane] = J0.rc2; tl[7][lane] = J0.rce; tl[8][lane] = J0.m;
+    __syncthreads();
     RevJAcc aj;
     rev_zero(aj);
     for (int t = 0; t < 64; ++t) {
-        const RevJ Jn = rev_next(J);
-        const int j = jt + ((lane + t) & 63);
+        const int slot = (lane + t) & 63;
+        RevJ J;
+        J.x[0] = tl[0][slot]; J.x[1] = tl[1][slot]; J.x[2] = tl[2][slot];
+        J.v[0] = tl[3][slot]; J.v[1] = tl[4][slot]; J.v[2] = tl[5][slot];
+        J.rc2 = tl[6][slot]; J.rce = tl[7][slot]; J.m = tl[8][slot];
+        const int j = jt + slot;
         const double Pm = (iok && j < jend && j != i) ? J.m : 0.0;
         rev_pair(I, J, U, Pm, ai, aj);
         rev_next(aj);
-        J = Jn;
     }
-    rev_jfin(J, U, aj, g);
+    rev_jfin(J0, U, aj, g);
 }
+
 
 // Block (s, ib): stars 1 + ib*WG + [0, WG), records [1 + s L, 1 + (s+1) L).
 // part rows: i side (s*NADJ1 + k)*cap + i, s < S; j side
@@ -381,9 +384,10 @@ __global__ __launch_bounds__(WG) void k_adj_rev(const ARec* __restrict__ rec,
     const int j0 = 1 + s * L, j1 = min(n, j0 + L);
     double* pj = part + (size_t)S * NADJ1 * cap;
     __shared__ double sh[NADJ2][4][64];
+    __shared__ double tls[4][REVJ_F][64];
     for (int jt = j0; jt < j1; jt += 64) {
         double g[NADJ2];
-        rev_tile(rec, I, U, i, iok, jt, j1, ai, g);
+        rev_tile(rec, I, U, i, iok, jt, j1, ai, g, tls[wv]);
 #pragma unroll
         for (int k = 0; k < NADJ2; ++k) sh[k][wv][lane] = g[k];
         __syncthreads();
