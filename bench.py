@@ -24,9 +24,11 @@ together through the ensemble API for --ens-steps steps (default 128), its
 interactions/s (sum of n^2 per force evaluation), pair-kernel roofline and
 mixed-vs-FP64 error, against the same clusters stepped one at a time, and
 the adjoint of every cluster's BH-mass gradient over the same steps,
-and an "adjoint" object: the differentiation workload (N = 16384, 256
-steps, checkpoints every 16) forward and backward wall times and the force
-VJP's pair rate (FP64).
+an "adjoint" object: the differentiation workload (N = 16384, 256 steps,
+checkpoints every 16) forward and backward wall times and the force VJP's
+pair rate (FP64), and a "shard_of_8" object: one GPU's share of a run whose
+stars are split across 8 GPUs (16384 stars against all 131073 particles,
+pn1_force_range), its pair kernel time and fraction of the full launch.
 """
 import argparse
 import json
