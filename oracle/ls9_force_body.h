@@ -25,6 +25,124 @@ static inline REAL FN(ref_rsqrt)(REAL r2) { return 1.0 / SQRT(r2); }
 /* ------------------------------------------------------------------ */
 /* force()  Limit-sticky9.c:21-424                                     */
 /* ------------------------------------------------------------------ */
+/* Per-star pieces of force(), shared by orc_force and orc_force_rows so
+ * that a row evaluated alone is bitwise that row of the full call.  Each
+ * *_term writes the bracket that the reference scales by lam and adds to
+ * (or subtracts from) a[i]; the caller applies lam, as the reference's
+ * a[i][k] += lam * (...) does. */
+
+/* Newtonian row i, j in [0,n) including self (:39-51). */
+static inline void FN(row_newton)(const REAL (*x)[3], const REAL *m, double eps2,
+                                  int i, int n, REAL s[3])
+{
+    REAL d[3];
+    s[0] = s[1] = s[2] = 0.0;
+    for (int j = 0; j < n; j++) {
+        for (int k = 0; k < 3; k++) d[k] = x[j][k] - x[i][k];
+        REAL r2 = FN(dot3)(d, d) + eps2;
+        REAL rinv = FN(ref_rsqrt)(r2);
+        REAL mr3 = (m[j] * rinv) * rinv * rinv;
+        for (int k = 0; k < 3; k++) s[k] += mr3 * d[k];
+    }
+}
+
+/* star <- BH 1PN, b = x0 - x_i (:215-238). */
+static inline void FN(row_star_bh_term)(const REAL (*x)[3], const REAL (*v)[3],
+                                        const REAL *m, double eps2, int i, REAL t[3])
+{
+    REAL db[3];
+    for (int k = 0; k < 3; k++) db[k] = x[0][k] - x[i][k];
+    REAL re = FN(ref_rsqrt)(FN(dot3)(db, db) + eps2);
+    REAL mri = m[0] * re;
+    REAL mr3 = mri * re * re;
+    REAL vv = FN(dot3)(v[i], v[i]);
+    REAL vd = FN(dot3)(v[i], db);
+    for (int k = 0; k < 3; k++)
+        t[k] = mr3 * db[k] * (4.0 * mri - vv) + 4.0 * mr3 * vd * v[i][k];
+}
+
+/* "aa_cross" single-star term with d = x_i - x0 (:249-274). */
+static inline void FN(row_aa_cross_term)(const REAL (*x)[3], const REAL (*v)[3],
+                                         const REAL *m, double eps2, int i, REAL t[3])
+{
+    REAL db[3];
+    for (int k = 0; k < 3; k++) db[k] = x[i][k] - x[0][k];
+    REAL re = FN(ref_rsqrt)(FN(dot3)(db, db) + eps2);
+    REAL mr3 = m[0] * re * re * re;
+    REAL mie3 = m[i] * re * re * re;
+    REAL vv = FN(dot3)(v[i], v[i]);
+    REAL vd = FN(dot3)(v[i], db);
+    for (int k = 0; k < 3; k++)
+        t[k] = 5.0 * mr3 * m[i] * re * db[k]
+               - mie3 * (4.0 * vv * db[k] - 7.0 * vd * v[i][k]);
+}
+
+/* ac three-body cross sum of star i, j = i included (:278-382). */
+static inline void FN(row_ac)(const REAL (*x)[3], const REAL (*v)[3], const REAL *m,
+                              double eps2, int i, int n, REAL s[3])
+{
+    REAL db[3], dc[3], dbc[3], dvbc[3];
+    s[0] = s[1] = s[2] = 0.0;
+    for (int j = 1; j < n; j++) {
+        for (int k = 0; k < 3; k++) {
+            db[k] = x[0][k] - x[i][k];
+            dc[k] = x[0][k] - x[j][k];
+            dbc[k] = x[j][k] - x[i][k];
+            dvbc[k] = v[j][k] - v[i][k];
+        }
+        REAL rb2 = FN(dot3)(db, db);
+        REAL rbe = FN(ref_rsqrt)(rb2 + eps2);
+        REAL rc2 = FN(dot3)(dc, dc);
+        REAL rce = FN(ref_rsqrt)(rc2 + eps2);
+        REAL rbc2 = FN(dot3)(dbc, dbc);
+        REAL rbc2e = rbc2 + eps2;
+        REAL rbce = FN(ref_rsqrt)(rbc2e);
+        REAL mr3 = (m[j] * rbce) * rbce * rbce;
+        /* (((((m_j rbe) rbe) rbe) m0) */
+        REAL c1 = m[j] * rbe * rbe * rbe * m[0];
+        REAL b1 = 4.0 * rbce + 1.25 * rce + 0.25 * (rb2 - rbc2) * rce * rce * rce;
+        REAL c2 = mr3 * m[0];
+        REAL b2 = 4.0 * rbe + 1.25 * rce + 0.25 * (-rb2 + rbc2) * rce * rce * rce;
+        REAL c3 = 3.5 * m[j] * m[0] * rce * rce * rce * (rbce - rbe);
+        REAL c4 = m[j] * rbe * rbe * rbe;
+        REAL vivj = FN(dot3)(v[i], v[j]);
+        REAL vjdb = FN(dot3)(v[j], db);
+        REAL vidb = FN(dot3)(v[i], db);
+        REAL vvi = FN(dot3)(v[i], v[i]);
+        REAL dvdv = FN(dot3)(dvbc, dvbc);
+        REAL vjr = FN(dot3)(v[j], dbc);
+        REAL b5 = vvi - 2.0 * dvdv + 1.5 * vjr * vjr / rbc2e;
+        /* ((r0 (4vi0 - 3vj0) + r1 (4vi1 - 3vj1)) + r2 (4vi2 - 3vj2)) */
+        REAL b6 = dbc[0] * (4.0 * v[i][0] - 3.0 * v[j][0])
+                    + dbc[1] * (4.0 * v[i][1] - 3.0 * v[j][1])
+                    + dbc[2] * (4.0 * v[i][2] - 3.0 * v[j][2]);
+        for (int k = 0; k < 3; k++)
+            s[k] += c1 * b1 * db[k] + c2 * b2 * dbc[k] - c3 * dc[k]
+                    - c4 * (4.0 * vivj * db[k] - 3.0 * vjdb * v[i][k]
+                            - 4.0 * vidb * v[j][k])
+                    + mr3 * b5 * dbc[k] + mr3 * b6 * dvbc[k];
+    }
+}
+
+/* the exact self-term removal of star i (:396-422), subtracted. */
+static inline void FN(row_self_term)(const REAL (*x)[3], const REAL (*v)[3],
+                                     const REAL *m, double eps, double eps2, int i,
+                                     REAL t[3])
+{
+    REAL db[3];
+    for (int k = 0; k < 3; k++) db[k] = x[0][k] - x[i][k];
+    REAL rb2 = FN(dot3)(db, db);
+    REAL rbe = FN(ref_rsqrt)(rb2 + eps2);
+    REAL mr3 = (m[i] * rbe) * rbe * rbe;
+    REAL vv = FN(dot3)(v[i], v[i]);
+    REAL vd = FN(dot3)(v[i], db);
+    REAL br = 4.0 / eps + 1.25 * rbe + 0.25 * rb2 * rbe * rbe * rbe;
+    for (int k = 0; k < 3; k++)
+        t[k] = mr3 * m[0] * db[k] * br
+               - 3.5 * mr3 * m[0] * (1.0 / eps - rbe) * db[k]
+               - mr3 * (4.0 * vv * db[k] - 7.0 * vd * v[i][k]);
+}
+
 void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
                    double eps, double c, REAL *as, int n)
 {
@@ -34,21 +152,11 @@ void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
     const double lam = 1.0 / (c * c);
     const double eps2 = eps * eps;
     REAL (*ac)[3] = xcalloc((size_t)n, sizeof *ac);
-    REAL d[3], db[3], dc[3], dbc[3], dvbc[3];
+    REAL d[3], db[3], dc[3], dbc[3], t[3];
     REAL acc1[3], a1c[3];
 
     /* Newtonian, all i, j in [0,n) including self (:39-51). */
-    for (int i = 0; i < n; i++) {
-        REAL s[3] = {0.0, 0.0, 0.0};
-        for (int j = 0; j < n; j++) {
-            for (int k = 0; k < 3; k++) d[k] = x[j][k] - x[i][k];
-            REAL r2 = FN(dot3)(d, d) + eps2;
-            REAL rinv = FN(ref_rsqrt)(r2);
-            REAL mr3 = (m[j] * rinv) * rinv * rinv;
-            for (int k = 0; k < 3; k++) s[k] += mr3 * d[k];
-        }
-        for (int k = 0; k < 3; k++) a[i][k] = s[k];
-    }
+    for (int i = 0; i < n; i++) FN(row_newton)(x, m, eps2, i, n, a[i]);
 
     /* a1: BH <- star 1PN, unsoftened /r2 (:54-97). */
     acc1[0] = acc1[1] = acc1[2] = 0.0;
@@ -119,93 +227,57 @@ void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
     }
     for (int k = 0; k < 3; k++) a[0][k] += lam * a1c[k];
 
-    /* star <- BH 1PN, b = x0 - x_i (:215-238). */
     for (int i = 1; i < n; i++) {
-        for (int k = 0; k < 3; k++) db[k] = x[0][k] - x[i][k];
-        REAL re = FN(ref_rsqrt)(FN(dot3)(db, db) + eps2);
-        REAL mri = m[0] * re;
-        REAL mr3 = mri * re * re;
-        REAL vv = FN(dot3)(v[i], v[i]);
-        REAL vd = FN(dot3)(v[i], db);
-        for (int k = 0; k < 3; k++)
-            a[i][k] += lam * (mr3 * db[k] * (4.0 * mri - vv) + 4.0 * mr3 * vd * v[i][k]);
+        FN(row_star_bh_term)(x, v, m, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[i][k] += lam * t[k];
     }
-
-    /* "aa_cross" single-star term with d = x_i - x0 (:249-274). */
     for (int i = 1; i < n; i++) {
-        for (int k = 0; k < 3; k++) db[k] = x[i][k] - x[0][k];
-        REAL re = FN(ref_rsqrt)(FN(dot3)(db, db) + eps2);
-        REAL mr3 = m[0] * re * re * re;
-        REAL mie3 = m[i] * re * re * re;
-        REAL vv = FN(dot3)(v[i], v[i]);
-        REAL vd = FN(dot3)(v[i], db);
-        for (int k = 0; k < 3; k++)
-            a[i][k] += lam * (5.0 * mr3 * m[i] * re * db[k]
-                              - mie3 * (4.0 * vv * db[k] - 7.0 * vd * v[i][k]));
+        FN(row_aa_cross_term)(x, v, m, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[i][k] += lam * t[k];
     }
-
-    /* ac three-body cross REAL sum, j = i included (:278-382). */
-    for (int i = 1; i < n; i++) {
-        REAL s[3] = {0.0, 0.0, 0.0};
-        for (int j = 1; j < n; j++) {
-            for (int k = 0; k < 3; k++) {
-                db[k] = x[0][k] - x[i][k];
-                dc[k] = x[0][k] - x[j][k];
-                dbc[k] = x[j][k] - x[i][k];
-                dvbc[k] = v[j][k] - v[i][k];
-            }
-            REAL rb2 = FN(dot3)(db, db);
-            REAL rbe = FN(ref_rsqrt)(rb2 + eps2);
-            REAL rc2 = FN(dot3)(dc, dc);
-            REAL rce = FN(ref_rsqrt)(rc2 + eps2);
-            REAL rbc2 = FN(dot3)(dbc, dbc);
-            REAL rbc2e = rbc2 + eps2;
-            REAL rbce = FN(ref_rsqrt)(rbc2e);
-            REAL mr3 = (m[j] * rbce) * rbce * rbce;
-            /* (((((m_j rbe) rbe) rbe) m0) */
-            REAL c1 = m[j] * rbe * rbe * rbe * m[0];
-            REAL b1 = 4.0 * rbce + 1.25 * rce + 0.25 * (rb2 - rbc2) * rce * rce * rce;
-            REAL c2 = mr3 * m[0];
-            REAL b2 = 4.0 * rbe + 1.25 * rce + 0.25 * (-rb2 + rbc2) * rce * rce * rce;
-            REAL c3 = 3.5 * m[j] * m[0] * rce * rce * rce * (rbce - rbe);
-            REAL c4 = m[j] * rbe * rbe * rbe;
-            REAL vivj = FN(dot3)(v[i], v[j]);
-            REAL vjdb = FN(dot3)(v[j], db);
-            REAL vidb = FN(dot3)(v[i], db);
-            REAL vvi = FN(dot3)(v[i], v[i]);
-            REAL dvdv = FN(dot3)(dvbc, dvbc);
-            REAL vjr = FN(dot3)(v[j], dbc);
-            REAL b5 = vvi - 2.0 * dvdv + 1.5 * vjr * vjr / rbc2e;
-            /* ((r0 (4vi0 - 3vj0) + r1 (4vi1 - 3vj1)) + r2 (4vi2 - 3vj2)) */
-            REAL b6 = dbc[0] * (4.0 * v[i][0] - 3.0 * v[j][0])
-                        + dbc[1] * (4.0 * v[i][1] - 3.0 * v[j][1])
-                        + dbc[2] * (4.0 * v[i][2] - 3.0 * v[j][2]);
-            for (int k = 0; k < 3; k++)
-                s[k] += c1 * b1 * db[k] + c2 * b2 * dbc[k] - c3 * dc[k]
-                        - c4 * (4.0 * vivj * db[k] - 3.0 * vjdb * v[i][k]
-                                - 4.0 * vidb * v[j][k])
-                        + mr3 * b5 * dbc[k] + mr3 * b6 * dvbc[k];
-        }
-        for (int k = 0; k < 3; k++) ac[i][k] = s[k];
-    }
+    for (int i = 1; i < n; i++) FN(row_ac)(x, v, m, eps2, i, n, ac[i]);
 
     /* add ac, then the exact self-term removal (:386-422). */
     for (int i = 1; i < n; i++)
         for (int k = 0; k < 3; k++) a[i][k] += lam * ac[i][k];
     for (int i = 1; i < n; i++) {
-        for (int k = 0; k < 3; k++) db[k] = x[0][k] - x[i][k];
-        REAL rb2 = FN(dot3)(db, db);
-        REAL rbe = FN(ref_rsqrt)(rb2 + eps2);
-        REAL mr3 = (m[i] * rbe) * rbe * rbe;
-        REAL vv = FN(dot3)(v[i], v[i]);
-        REAL vd = FN(dot3)(v[i], db);
-        REAL br = 4.0 / eps + 1.25 * rbe + 0.25 * rb2 * rbe * rbe * rbe;
-        for (int k = 0; k < 3; k++)
-            a[i][k] -= lam * (mr3 * m[0] * db[k] * br
-                              - 3.5 * mr3 * m[0] * (1.0 / eps - rbe) * db[k]
-                              - mr3 * (4.0 * vv * db[k] - 7.0 * vd * v[i][k]));
+        FN(row_self_term)(x, v, m, eps, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[i][k] -= lam * t[k];
     }
     free(ac);
+}
+
+/* Rows rows[0..nrows) (stars, 1 <= rows[r] < n) of force(): out[r] is
+ * bitwise a[rows[r]] of orc_force on the same input.  A star's row is
+ * O(n); the BH's row needs every pair and is not offered here. */
+void FN(orc_force_rows)(const REAL *xs, const REAL *vs, const REAL *m,
+                        double eps, double c, const int *rows, int nrows,
+                        REAL *outs, int n)
+{
+    const REAL (*x)[3] = (const REAL (*)[3])xs;
+    const REAL (*v)[3] = (const REAL (*)[3])vs;
+    REAL (*out)[3] = (REAL (*)[3])outs;
+    const double lam = 1.0 / (c * c);
+    const double eps2 = eps * eps;
+    REAL a[3], t[3], s[3];
+
+    for (int r = 0; r < nrows; r++) {
+        int i = rows[r];
+        if (i < 1 || i >= n) {
+            fprintf(stderr, "ls9_oracle: row %d outside the stars [1, %d)\n", i, n);
+            exit(1);
+        }
+        FN(row_newton)(x, m, eps2, i, n, a);
+        FN(row_star_bh_term)(x, v, m, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[k] += lam * t[k];
+        FN(row_aa_cross_term)(x, v, m, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[k] += lam * t[k];
+        FN(row_ac)(x, v, m, eps2, i, n, s);
+        for (int k = 0; k < 3; k++) a[k] += lam * s[k];
+        FN(row_self_term)(x, v, m, eps, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[k] -= lam * t[k];
+        for (int k = 0; k < 3; k++) out[r][k] = a[k];
+    }
 }
 
 /* ------------------------------------------------------------------ */

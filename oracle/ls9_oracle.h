@@ -12,6 +12,11 @@
 /* force()  Limit-sticky9.c:21-424 (a is overwritten) */
 void orc_force(const double *x, const double *v, const double *m,
                double eps, double c, double *a, int n);
+/* rows[0..nrows) (stars, 1 <= rows[r] < n) of force(); out is [nrows][3]
+ * and bitwise those rows of orc_force */
+void orc_force_rows(const double *x, const double *v, const double *m,
+                    double eps, double c, const int *rows, int nrows,
+                    double *out, int n);
 void orc_force_only(const double *x, const double *v, const double *m,
                     double eps, double c, double *a, int n);
 

@@ -34,6 +34,8 @@ def oracle():
                            check=True)
         L = ctypes.CDLL(ORACLE_LIB)
         L.orc_force.argtypes = [_D, _D, _D, _dbl, _dbl, _D, _int]
+        L.orc_force_rows.argtypes = [_D, _D, _D, _dbl, _dbl, ctypes.POINTER(_int), _int,
+                                     _D, _int]
         L.orc_energy_terms.argtypes = [_D, _D, _D, _int, _dbl, _dbl, _D]
         L.orc_initial_energy.argtypes = [_D, _D, _D, _int, _D, _dbl, _dbl]
         L.orc_energy.argtypes = [_dbl, _D, _D, _D, _int, _dbl, _dbl, _dbl]
@@ -118,6 +120,18 @@ def orc_force(x, v, m, eps, c=1.0):
     a = np.zeros_like(x)
     oracle().orc_force(ptr(x), ptr(v), ptr(m), eps, c, ptr(a), m.shape[0])
     return a
+
+
+def orc_force_rows(x, v, m, eps, c, rows):
+    """Rows `rows` (stars) of orc_force, each O(n): bitwise those rows of the
+    full call."""
+    x, v, m = _arr(x, v, m)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.zeros((rows.shape[0], 3))
+    oracle().orc_force_rows(ptr(x), ptr(v), ptr(m), eps, c,
+                            rows.ctypes.data_as(ctypes.POINTER(_int)), rows.shape[0],
+                            ptr(out), m.shape[0])
+    return out
 
 
 def orc_energy_terms(x, v, m, eps, c=1.0):

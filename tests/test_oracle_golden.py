@@ -12,6 +12,22 @@ def test_force_c1_bitwise(force_fixture):
     assert np.array_equal(a.view(np.uint64), d["a_c1"].view(np.uint64)), d["name"]
 
 
+def test_force_rows_bitwise(force_fixture):
+    """orc_force_rows (the O(n)-per-star rows the n = 270000 GPU test checks
+    against) equals the golden vectors' star rows bit for bit at c = 1 and
+    orc_force's at the fixture's other c, rows taken in a shuffled order."""
+    d = force_fixture
+    n = d["m"].shape[0]
+    if n < 2:
+        pytest.skip("no star rows")
+    rows = np.random.default_rng(n).permutation(np.arange(1, n))
+    for c in [1.0] + c_values(d):
+        full = d["a_c1"] if c == 1.0 else ob.orc_force(d["x"], d["v"], d["m"],
+                                                       float(d["eps"]), c)
+        got = ob.orc_force_rows(d["x"], d["v"], d["m"], float(d["eps"]), c, rows)
+        assert np.array_equal(got.view(np.uint64), full[rows].view(np.uint64)), (d["name"], c)
+
+
 def test_initial_energy_c1_bitwise_and_text(force_fixture):
     d = force_fixture
     E0, txt = ob.orc_initial_energy_text(d["x"], d["v"], d["m"], float(d["eps"]), 1.0)
