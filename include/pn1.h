@@ -35,6 +35,11 @@ extern "C" {
 #define PN1_FP64   0
 #define PN1_MIXED  1
 
+/* Largest capacity a context or ensemble accepts: the kernels index the
+ * six state rows of `capacity` doubles with 32-bit offsets (6 x 2^28 <
+ * 2^31); at ~0.5 KB of device state per particle it is also ~130 GB. */
+#define PN1_MAX_CAPACITY (1 << 28)
+
 /* ====================================================================== */
 /* 1. Legacy shapes: drop-in replacements for the reference's functions.  */
 /*    Same prototypes (the reference's [NMAX][3] parameters decay to      */
@@ -80,7 +85,8 @@ typedef struct {
     double total;    /* kin + pot + pot_pn + pot_pn2                        */
 } pn1_energy_t;
 
-/* Creates a context for up to `capacity` particles on HIP device `device`. */
+/* Creates a context for up to `capacity` particles on HIP device `device`
+ * (1 <= capacity <= PN1_MAX_CAPACITY). */
 int  pn1_create(pn1_ctx **out, int capacity, int device);
 void pn1_destroy(pn1_ctx *ctx);
 /* eps > 0 (Plummer softening), c > 0, dt > 0, mode PN1_FP64 | PN1_MIXED. */
@@ -184,7 +190,8 @@ int  pn1_adjoint_backward(pn1_ctx *ctx, const double *gx, const double *gv,
  * items run in. */
 typedef struct pn1_ens pn1_ens;
 
-/* Up to `capacity` particles in total in up to `max_clusters` clusters. */
+/* Up to `capacity` particles in total (at most PN1_MAX_CAPACITY) in up to
+ * `max_clusters` clusters. */
 int  pn1_ens_create(pn1_ens **out, int capacity, int max_clusters,
                     int device);
 void pn1_ens_destroy(pn1_ens *ens);

@@ -295,7 +295,7 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
 {
     if (!out) return PN1_EINVAL;
     *out = nullptr;
-    if (capacity < 1) return PN1_EINVAL;
+    if (capacity < 1 || capacity > PN1_MAX_CAPACITY) return PN1_EINVAL;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PN1_ENODEV;
     if (device < 0 || device >= ndev) return PN1_ENODEV;

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libpn1.so")
 
 PN1_OK, PN1_EINVAL, PN1_ENOMEM, PN1_EDEVICE, PN1_ENODEV, PN1_EIO = 0, -1, -2, -3, -4, -5
 FP64, MIXED = 0, 1
+PN1_MAX_CAPACITY = 1 << 28
 _STATUS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ENODEV", -5: "EIO"}
 
 _D = ctypes.POINTER(ctypes.c_double)

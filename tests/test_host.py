@@ -182,6 +182,19 @@ def test_no_device_fails_loudly():
     assert r.returncode == 1 and "gfx950 device" in r.stderr, r.stderr
 
 
+def test_capacity_bounds_rejected_before_device():
+    """Capacities outside [1, PN1_MAX_CAPACITY] are refused with EINVAL before
+    any device is touched (the kernels' 32-bit state offsets stop there)."""
+    for cap in (0, -5, pn.PN1_MAX_CAPACITY + 1, 2**31 - 1):
+        with pytest.raises(pn.Pn1Error) as ei:
+            pn.Context(cap)
+        assert ei.value.code == pn.PN1_EINVAL, cap
+        with pytest.raises(pn.Pn1Error) as ei:
+            pn.Ensemble(cap, 4)
+        assert ei.value.code == pn.PN1_EINVAL, cap
+    assert 6 * pn.PN1_MAX_CAPACITY < 2**31
+
+
 def test_sticky9_missing_input_and_usage(tmp_path):
     r = subprocess.run([STICKY9, "--in", str(tmp_path / "none.dat"), "--out",
                         str(tmp_path / "o.dat")], capture_output=True, text=True, timeout=60)
@@ -197,3 +210,5 @@ def test_status_codes_match_header():
                       ("PN1_ENODEV", pn.PN1_ENODEV), ("PN1_EIO", pn.PN1_EIO),
                       ("PN1_FP64", pn.FP64), ("PN1_MIXED", pn.MIXED)):
         assert int(re.search(rf"#define {name}\s+\(?(-?\d+)\)?", text).group(1)) == val
+    sh = re.search(r"#define PN1_MAX_CAPACITY\s+\(1 << (\d+)\)", text)
+    assert 1 << int(sh.group(1)) == pn.PN1_MAX_CAPACITY
