@@ -286,3 +286,47 @@ def test_gpu_matches_oracle_on_random_small_systems():
         check()
     finally:
         ctx.close()
+
+
+def test_distinct_contexts_in_threads():
+    """Contexts are independent: four host threads, each driving its own
+    context (force, graph-captured steps, energy), give bitwise the results
+    of the same work run one after another.  ctypes drops the GIL during
+    each library call, so the calls overlap."""
+    import threading
+
+    cases = [(1025, 21, pn.FP64), (4097, 22, pn.MIXED), (3001, 23, pn.FP64),
+             (257, 24, pn.MIXED)]
+
+    def work(n, seed, mode):
+        x, v, m = pn.ic_uniform_sphere(n - 1, seed, R=1.0, Q=0.5, m_bh=0.01, eps=1e-3)
+        ctx = pn.Context(n)
+        try:
+            ctx.set_params(1e-3, 100.0, 2.0**-10, mode)
+            ctx.upload(x, v, m)
+            a = ctx.force()
+            ctx.step(3)
+            xs, vs = ctx.download()
+            return a, xs, vs, ctx.energy()
+        finally:
+            ctx.close()
+
+    serial = [work(*cs) for cs in cases]
+    got, errs = [None] * len(cases), []
+
+    def run(k):
+        try:
+            got[k] = work(*cases[k])
+        except BaseException as ex:  # reported below
+            errs.append((k, ex))
+
+    ts = [threading.Thread(target=run, args=(k,)) for k in range(len(cases))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in ts), "a context thread did not finish"
+    assert not errs, errs
+    for k, (s, g) in enumerate(zip(serial, got)):
+        for p, q in zip(s, g):
+            assert np.array_equal(np.asarray(p), np.asarray(q)), cases[k]
