@@ -53,6 +53,7 @@ FLOPS_PER_INTERACTION = 90
 PEAK_FP32 = 157.3e12   # MI355X vector FP32
 PEAK_FP64 = 78.6e12    # MI355X vector FP64
 CPU_SAMPLE_STARS = 24576
+CPU_OMP_SAMPLE_STARS = 49152
 ENS_TRAFFIC = "ensemble_pair_traffic_c5.json"
 
 
@@ -170,6 +171,31 @@ def cpu_baseline(x, v, m):
     return {"value": float(n) * n / dt, "unit": "interactions/s", "cores": 1, "kind": kind,
             "sample": f"one force() call, first {CPU_SAMPLE_STARS} stars + BH of the workload "
                       f"(n={n}, {n * n:.3e} interactions, {dt:.1f} s, gcc -O2, 1 thread, {cpu})"}
+
+
+def cpu_openmp(x, v, m):
+    """Beside the one-core baseline: the oracle's force() restatement with its
+    O(n^2) loops over OpenMP threads (one per core of this process's CPU
+    share, at most 16), on the first CPU_OMP_SAMPLE_STARS stars.  Not the
+    reference (which is serial); its star rows are bitwise the serial ones."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_bind as ob
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    n = CPU_OMP_SAMPLE_STARS + 1
+    xs, vs, ms = x[:n].copy(), v[:n].copy(), m[:n].copy()
+    t0 = time.perf_counter()
+    _, ran = ob.orc_force_omp(xs, vs, ms, EPS, C_LIGHT, threads)
+    dt = time.perf_counter() - t0
+    return {"value": float(n) * n / dt, "unit": "interactions/s", "cores": int(ran),
+            "kind": "port",
+            "sample": f"one force() call of the oracle restatement over {ran} OpenMP threads, "
+                      f"first {CPU_OMP_SAMPLE_STARS} stars + BH of the workload (n={n}, "
+                      f"{n * n:.3e} interactions, {dt:.1f} s, gcc -O2 -fopenmp)"}
 
 
 def time_steps(obj, steps, warmup, eager=False):
@@ -533,6 +559,7 @@ def main():
         t.join()
         threading.stack_size(old)
         out["cpu_baseline"] = res.get("r")
+        out["cpu_openmp"] = cpu_openmp(x, v, m)
     print(json.dumps(out), flush=True)
 
 

@@ -143,22 +143,15 @@ static inline void FN(row_self_term)(const REAL (*x)[3], const REAL (*v)[3],
                - mr3 * (4.0 * vv * db[k] - 7.0 * vd * v[i][k]);
 }
 
-void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
-                   double eps, double c, REAL *as, int n)
+/* The BH's 1PN sums.  bh_a1c_cross_row adds the j loop of one i to the
+ * caller's accumulator, so a serial caller looping i = 1 .. n-1 keeps the
+ * reference's single-accumulator order (i outer, j inner). */
+
+/* a1: BH <- star 1PN, unsoftened /r2 (:54-97). */
+static inline void FN(bh_a1)(const REAL (*x)[3], const REAL (*v)[3], const REAL *m,
+                             double eps2, int n, REAL acc1[3])
 {
-    const REAL (*x)[3] = (const REAL (*)[3])xs;
-    const REAL (*v)[3] = (const REAL (*)[3])vs;
-    REAL (*a)[3] = (REAL (*)[3])as;
-    const double lam = 1.0 / (c * c);
-    const double eps2 = eps * eps;
-    REAL (*ac)[3] = xcalloc((size_t)n, sizeof *ac);
-    REAL d[3], db[3], dc[3], dbc[3], t[3];
-    REAL acc1[3], a1c[3];
-
-    /* Newtonian, all i, j in [0,n) including self (:39-51). */
-    for (int i = 0; i < n; i++) FN(row_newton)(x, m, eps2, i, n, a[i]);
-
-    /* a1: BH <- star 1PN, unsoftened /r2 (:54-97). */
+    REAL d[3];
     acc1[0] = acc1[1] = acc1[2] = 0.0;
     for (int i = 1; i < n; i++) {
         for (int k = 0; k < 3; k++) d[k] = x[i][k] - x[0][k];
@@ -172,9 +165,13 @@ void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
         for (int k = 0; k < 3; k++)
             acc1[k] += mr3 * d[k] * br + 3.0 * mr3 * v[i][k] * vd;
     }
-    for (int k = 0; k < 3; k++) a[0][k] += lam * acc1[k];
+}
 
-    /* a1c prefix 4 m_i^2 e^4 d (:101-115). */
+/* a1c prefix 4 m_i^2 e^4 d (:101-115); starts the a1c accumulator. */
+static inline void FN(bh_a1c_prefix)(const REAL (*x)[3], const REAL *m, double eps2, int n,
+                                     REAL a1c[3])
+{
+    REAL db[3];
     a1c[0] = a1c[1] = a1c[2] = 0.0;
     for (int i = 1; i < n; i++) {
         for (int k = 0; k < 3; k++) db[k] = x[i][k] - x[0][k];
@@ -182,39 +179,48 @@ void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
         REAL mr3 = (m[i] * re) * re * re;
         for (int k = 0; k < 3; k++) a1c[k] += 4.0 * mr3 * re * m[i] * db[k];
     }
+}
 
-    /* a1c three-body cross REAL sum, j = i included (:118-174). */
-    for (int i = 1; i < n; i++) {
-        for (int j = 1; j < n; j++) {
-            for (int k = 0; k < 3; k++) {
-                db[k] = x[i][k] - x[0][k];
-                dc[k] = x[j][k] - x[0][k];
-                dbc[k] = x[j][k] - x[i][k];
-            }
-            REAL rb2 = FN(dot3)(db, db);
-            REAL rbe = FN(ref_rsqrt)(rb2 + eps2);
-            REAL rc2 = FN(dot3)(dc, dc);
-            REAL rce = FN(ref_rsqrt)(rc2 + eps2);
-            REAL rbc2e = FN(dot3)(dbc, dbc) + eps2;
-            REAL rbce = FN(ref_rsqrt)(rbc2e);
-            REAL mr3 = (m[i] * rbe) * rbe * rbe;
-            REAL vivj = FN(dot3)(v[i], v[j]);
-            REAL vidb = FN(dot3)(v[i], db);
-            REAL vjdb = FN(dot3)(v[j], db);
-            /* ((4 rce + 1.25 rbce) - ((0.25 rc2)/rbc2e) rbce) + ((0.25 rb2)/rbc2e) rbce */
-            REAL br = 4.0 * rce + 1.25 * rbce - 0.25 * rc2 / rbc2e * rbce
-                        + 0.25 * rb2 / rbc2e * rbce;
-            /* ((((3.5 (rbce rbce rbce)) rbe) m_i) m_j) */
-            REAL cn = 3.5 * (rbce * rbce * rbce) * rbe * m[i] * m[j];
-            REAL cv = mr3 * m[j] / m[0];
-            for (int k = 0; k < 3; k++)
-                a1c[k] += mr3 * m[j] * db[k] * br - cn * dbc[k]
-                          - cv * (4.0 * vivj * db[k] - 3.0 * vidb * v[j][k]
-                                  - 4.0 * vjdb * v[i][k]);
+/* a1c three-body cross sum, row i, j = i included (:118-174). */
+static inline void FN(bh_a1c_cross_row)(const REAL (*x)[3], const REAL (*v)[3],
+                                        const REAL *m, double eps2, int i, int n,
+                                        REAL a1c[3])
+{
+    REAL db[3], dc[3], dbc[3];
+    for (int j = 1; j < n; j++) {
+        for (int k = 0; k < 3; k++) {
+            db[k] = x[i][k] - x[0][k];
+            dc[k] = x[j][k] - x[0][k];
+            dbc[k] = x[j][k] - x[i][k];
         }
+        REAL rb2 = FN(dot3)(db, db);
+        REAL rbe = FN(ref_rsqrt)(rb2 + eps2);
+        REAL rc2 = FN(dot3)(dc, dc);
+        REAL rce = FN(ref_rsqrt)(rc2 + eps2);
+        REAL rbc2e = FN(dot3)(dbc, dbc) + eps2;
+        REAL rbce = FN(ref_rsqrt)(rbc2e);
+        REAL mr3 = (m[i] * rbe) * rbe * rbe;
+        REAL vivj = FN(dot3)(v[i], v[j]);
+        REAL vidb = FN(dot3)(v[i], db);
+        REAL vjdb = FN(dot3)(v[j], db);
+        /* ((4 rce + 1.25 rbce) - ((0.25 rc2)/rbc2e) rbce) + ((0.25 rb2)/rbc2e) rbce */
+        REAL br = 4.0 * rce + 1.25 * rbce - 0.25 * rc2 / rbc2e * rbce
+                    + 0.25 * rb2 / rbc2e * rbce;
+        /* ((((3.5 (rbce rbce rbce)) rbe) m_i) m_j) */
+        REAL cn = 3.5 * (rbce * rbce * rbce) * rbe * m[i] * m[j];
+        REAL cv = mr3 * m[j] / m[0];
+        for (int k = 0; k < 3; k++)
+            a1c[k] += mr3 * m[j] * db[k] * br - cn * dbc[k]
+                      - cv * (4.0 * vivj * db[k] - 3.0 * vidb * v[j][k]
+                              - 4.0 * vjdb * v[i][k]);
     }
+}
 
-    /* a1c "remove REAL-counting", as written (:179-207). */
+/* a1c "remove REAL-counting", as written (:179-207). */
+static inline void FN(bh_a1c_removal)(const REAL (*x)[3], const REAL (*v)[3], const REAL *m,
+                                      double eps, double eps2, int n, REAL a1c[3])
+{
+    REAL db[3];
     for (int i = 1; i < n; i++) {
         for (int k = 0; k < 3; k++) db[k] = x[0][k] - x[i][k];
         REAL re = FN(ref_rsqrt)(FN(dot3)(db, db) + eps2);
@@ -225,6 +231,27 @@ void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
             a1c[k] -= mr3 * m[i] * db[k] * (4.0 * re + 1.25 * eps)
                       - mr3 * m[i] / m[0] * (4.0 * vv * db[k] - 7.0 * vd * v[i][k]);
     }
+}
+
+void FN(orc_force)(const REAL *xs, const REAL *vs, const REAL *m,
+                   double eps, double c, REAL *as, int n)
+{
+    const REAL (*x)[3] = (const REAL (*)[3])xs;
+    const REAL (*v)[3] = (const REAL (*)[3])vs;
+    REAL (*a)[3] = (REAL (*)[3])as;
+    const double lam = 1.0 / (c * c);
+    const double eps2 = eps * eps;
+    REAL (*ac)[3] = xcalloc((size_t)n, sizeof *ac);
+    REAL t[3], acc1[3], a1c[3];
+
+    /* Newtonian, all i, j in [0,n) including self (:39-51). */
+    for (int i = 0; i < n; i++) FN(row_newton)(x, m, eps2, i, n, a[i]);
+
+    FN(bh_a1)(x, v, m, eps2, n, acc1);
+    for (int k = 0; k < 3; k++) a[0][k] += lam * acc1[k];
+    FN(bh_a1c_prefix)(x, m, eps2, n, a1c);
+    for (int i = 1; i < n; i++) FN(bh_a1c_cross_row)(x, v, m, eps2, i, n, a1c);
+    FN(bh_a1c_removal)(x, v, m, eps, eps2, n, a1c);
     for (int k = 0; k < 3; k++) a[0][k] += lam * a1c[k];
 
     for (int i = 1; i < n; i++) {

@@ -29,6 +29,7 @@
  * the n = 1 and n = 2 closed forms, and the invariants in tests/.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -173,6 +174,60 @@ void orc_energy(double t, const double *x, const double *v, const double *m,
     double de = (init_ene - e[4]) / e[4];
     print_energy_block(t, e);
     printf(" DE = %e %g\n", de, t);
+}
+
+/* force() with its O(n^2) loops spread over `threads` OpenMP threads: a CPU
+ * baseline on all of a host's cores (not the reference, which is serial).
+ * Star rows are bitwise those of orc_force (each row is one thread's serial
+ * loop, in the reference's order); the BH's three-body a1c sum is split
+ * into per-thread partials added in thread order, so a[0] differs from the
+ * serial sum in the last bits.  Returns the number of threads that ran. */
+int orc_force_omp(const double *xs, const double *vs, const double *m, double eps, double c,
+                  double *as, int n, int threads)
+{
+    const double (*x)[3] = (const double (*)[3])xs;
+    const double (*v)[3] = (const double (*)[3])vs;
+    double (*a)[3] = (double (*)[3])as;
+    const double lam = 1.0 / (c * c);
+    const double eps2 = eps * eps;
+    double acc1[3], a1c[3];
+    if (threads < 1) threads = 1;
+    double (*part)[3] = xcalloc((size_t)threads, sizeof *part);
+    int ran = 1;
+
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int i = 0; i < n; i++) row_newton(x, m, eps2, i, n, a[i]);
+
+    bh_a1(x, v, m, eps2, n, acc1);
+    for (int k = 0; k < 3; k++) a[0][k] += lam * acc1[k];
+    bh_a1c_prefix(x, m, eps2, n, a1c);
+#pragma omp parallel num_threads(threads)
+    {
+        const int t = omp_get_thread_num();
+#pragma omp single
+        ran = omp_get_num_threads();
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 1; i < n; i++) bh_a1c_cross_row(x, v, m, eps2, i, n, part[t]);
+    }
+    for (int t = 0; t < threads; t++)
+        for (int k = 0; k < 3; k++) a1c[k] += part[t][k];
+    bh_a1c_removal(x, v, m, eps, eps2, n, a1c);
+    for (int k = 0; k < 3; k++) a[0][k] += lam * a1c[k];
+
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int i = 1; i < n; i++) {
+        double t[3], s[3];
+        row_star_bh_term(x, v, m, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[i][k] += lam * t[k];
+        row_aa_cross_term(x, v, m, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[i][k] += lam * t[k];
+        row_ac(x, v, m, eps2, i, n, s);
+        for (int k = 0; k < 3; k++) a[i][k] += lam * s[k];
+        row_self_term(x, v, m, eps, eps2, i, t);
+        for (int k = 0; k < 3; k++) a[i][k] -= lam * t[k];
+    }
+    free(part);
+    return ran;
 }
 
 /* One force evaluation, timed by the caller: a plain wrapper so a CPU

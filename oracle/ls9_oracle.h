@@ -17,6 +17,11 @@ void orc_force(const double *x, const double *v, const double *m,
 void orc_force_rows(const double *x, const double *v, const double *m,
                     double eps, double c, const int *rows, int nrows,
                     double *out, int n);
+/* force() over `threads` OpenMP threads (a multi-core CPU baseline): star
+ * rows bitwise those of orc_force, a[0] to the reordering of one sum;
+ * returns the number of threads that ran */
+int orc_force_omp(const double *x, const double *v, const double *m, double eps,
+                  double c, double *a, int n, int threads);
 void orc_force_only(const double *x, const double *v, const double *m,
                     double eps, double c, double *a, int n);
 

@@ -34,6 +34,8 @@ def oracle():
                            check=True)
         L = ctypes.CDLL(ORACLE_LIB)
         L.orc_force.argtypes = [_D, _D, _D, _dbl, _dbl, _D, _int]
+        L.orc_force_omp.argtypes = [_D, _D, _D, _dbl, _dbl, _D, _int, _int]
+        L.orc_force_omp.restype = _int
         L.orc_force_rows.argtypes = [_D, _D, _D, _dbl, _dbl, ctypes.POINTER(_int), _int,
                                      _D, _int]
         L.orc_energy_terms.argtypes = [_D, _D, _D, _int, _dbl, _dbl, _D]
@@ -120,6 +122,14 @@ def orc_force(x, v, m, eps, c=1.0):
     a = np.zeros_like(x)
     oracle().orc_force(ptr(x), ptr(v), ptr(m), eps, c, ptr(a), m.shape[0])
     return a
+
+
+def orc_force_omp(x, v, m, eps, c, threads):
+    """force() over OpenMP threads: (a, number of threads that ran)."""
+    x, v, m = _arr(x, v, m)
+    a = np.zeros_like(x)
+    ran = oracle().orc_force_omp(ptr(x), ptr(v), ptr(m), eps, c, ptr(a), m.shape[0], threads)
+    return a, ran
 
 
 def orc_force_rows(x, v, m, eps, c, rows):

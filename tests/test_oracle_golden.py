@@ -28,6 +28,20 @@ def test_force_rows_bitwise(force_fixture):
         assert np.array_equal(got.view(np.uint64), full[rows].view(np.uint64)), (d["name"], c)
 
 
+@pytest.mark.parametrize("name", ["rand_n257", "c1_n1025", "close_n65"])
+def test_force_omp_matches_serial(name):
+    """The OpenMP force (the multi-core CPU baseline): star rows bitwise the
+    golden vectors', the BH's row to the reordering of its a1c pair sum."""
+    d = load_fixture(name)
+    for c in [1.0] + c_values(d):
+        full = d["a_c1"] if c == 1.0 else ob.orc_force(d["x"], d["v"], d["m"],
+                                                       float(d["eps"]), c)
+        a, ran = ob.orc_force_omp(d["x"], d["v"], d["m"], float(d["eps"]), c, 4)
+        assert ran >= 1
+        assert np.array_equal(a[1:].view(np.uint64), full[1:].view(np.uint64)), (name, c)
+        assert np.abs(a[0] - full[0]).max() <= 1e-12 * np.abs(full[0]).max(), (name, c)
+
+
 def test_initial_energy_c1_bitwise_and_text(force_fixture):
     d = force_fixture
     E0, txt = ob.orc_initial_energy_text(d["x"], d["v"], d["m"], float(d["eps"]), 1.0)
