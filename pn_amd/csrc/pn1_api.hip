@@ -209,17 +209,10 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
                        mixed ? c->j32 : nullptr, c->gblk);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     if (c->n > 1) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (c->prof) {
-            HIPCHK(c, hipEventCreate(&e0));
-            HIPCHK(c, hipEventCreate(&e1));
-            HIPCHK(c, hipEventRecord(e0, c->st));
-        }
+        hipEvent_t e1 = nullptr;
+        if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
         launch_pair(c, in, P, 0, c->nIblk, c->S);
-        if (c->prof) {
-            HIPCHK(c, hipEventRecord(e1, c->st));
-            c->ev.emplace_back(e0, e1);
-        }
+        if (e1) HIPCHK(c, hipEventRecord(e1, c->st));
     }
     switch (stage * 2 + (mixed ? 1 : 0)) {
     case 0: launch_fin<0, 0>(c, in, P); break;
@@ -466,17 +459,10 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
                        mixed ? c->j32 : nullptr, c->gblk);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->prof) {
-        HIPCHK(c, hipEventCreate(&e0));
-        HIPCHK(c, hipEventCreate(&e1));
-        HIPCHK(c, hipEventRecord(e0, c->st));
-    }
+    hipEvent_t e1 = nullptr;
+    if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
     launch_pair(c, in, P, (i0 - 1) / WG, nib, S);
-    if (c->prof) {
-        HIPCHK(c, hipEventRecord(e1, c->st));
-        c->ev.emplace_back(e0, e1);
-    }
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->st));
     const dim3 gf(nib);
     if (mixed)
         hipLaunchKernelGGL((k_fin<1, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,

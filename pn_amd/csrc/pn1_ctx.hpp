@@ -59,6 +59,24 @@ void adj_free(pn1_ctx* c);   // pn1_adj.hip
 
 }  // namespace pn1host
 
+// Opens a timed region on stream `st`: two events handed to `ev` at once
+// (so a later error leaks neither), the first recorded now; the caller
+// records *end after the launches it times.
+inline hipError_t prof_open(std::vector<std::pair<hipEvent_t, hipEvent_t>>& ev, hipStream_t st,
+                            hipEvent_t* end)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t r = hipEventCreate(&a);
+    if (r != hipSuccess) return r;
+    if ((r = hipEventCreate(&b)) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return r;
+    }
+    ev.emplace_back(a, b);
+    *end = b;
+    return hipEventRecord(a, st);
+}
+
 #define HIPCHK(ctx, call)                                                                 \
     do {                                                                                  \
         hipError_t e_ = (call);                                                           \
