@@ -73,7 +73,7 @@ namespace {
 template <typename T>
 int aalloc(pn1_ctx* c, T** p, size_t count)
 {
-    if (hipMalloc((void**)p, (count ? count : 1) * sizeof(T)) != hipSuccess) {
+    if (dev_malloc((void**)p, (count ? count : 1) * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes (adjoint) failed", count * sizeof(T));
     }
@@ -247,6 +247,9 @@ int pn1_adjoint_forward(pn1_ctx* c, int steps, int interval)
         return seterr(c, PN1_EINVAL, "steps (%d) and interval (%d) must be >= 1", steps, interval);
     pn1_adj* a;
     if ((rc = adj_get(c, &a))) return rc;
+    // a run that stops below leaves nothing for pn1_adjoint_backward to
+    // reverse (its checkpoint buffers may have been replaced)
+    a->have_fwd = false;
     const int nck = (steps + interval - 1) / interval;
     const size_t s6 = 6 * (size_t)c->cap;
     if (nck > a->nck_cap) {
@@ -528,7 +531,7 @@ namespace {
 template <typename T>
 int ealloc_adj(pn1_ens* e, T** p, size_t count)
 {
-    if (hipMalloc((void**)p, (count ? count : 1) * sizeof(T)) != hipSuccess) {
+    if (dev_malloc((void**)p, (count ? count : 1) * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return eerr(e, PN1_ENOMEM, "hipMalloc of %zu bytes (adjoint) failed", count * sizeof(T));
     }
@@ -717,6 +720,7 @@ int pn1_ens_adjoint_forward(pn1_ens* e, int steps, int interval)
         return eerr(e, PN1_EINVAL, "steps (%d) and interval (%d) must be >= 1", steps, interval);
     pn1_ens_adj* a;
     if ((rc = ens_adj_get(e, &a))) return rc;
+    a->have_fwd = false;                 // as in pn1_adjoint_forward
     const int nck = (steps + interval - 1) / interval;
     const size_t s6 = 6 * (size_t)e->cap;
     if (nck > a->nck_cap) {

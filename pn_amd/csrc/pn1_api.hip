@@ -49,7 +49,7 @@ int seterr(pn1_ctx* c, int code, const char* fmt, ...)
 template <typename T>
 int dalloc(pn1_ctx* c, T** p, size_t count)
 {
-    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+    if (dev_malloc((void**)p, count * sizeof(T)) != hipSuccess) {
         *p = nullptr;
         return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes failed", count * sizeof(T));
     }
@@ -116,7 +116,7 @@ int ensure_part(pn1_ctx* c, int split = 0)
     if (c->part) (void)hipFree(c->part);
     c->part = nullptr;
     c->part_bytes = 0;
-    if (hipMalloc(&c->part, need) != hipSuccess) {
+    if (dev_malloc(&c->part, need) != hipSuccess) {
         c->part = nullptr;
         return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes for pair partials failed", need);
     }
@@ -324,7 +324,7 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
     // two records more than particles: the pair loops fetch up to jr[j + 2]
     if (!rc) rc = dalloc(c, &c->j64, cap + 2);
     if (!rc) rc = dalloc(c, &c->j32, cap + 2);
-    if (!rc && hipHostMalloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
+    if (!rc && host_malloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
         c->h_stage = nullptr;
         rc = PN1_ENOMEM;
     }
@@ -673,8 +673,8 @@ int pn1_util_copy(pn1_ctx* c, long long bytes, int reps)
     HIPCHK(c, hipSetDevice(c->dev));
     const size_t n2 = (size_t)bytes / 16;
     void *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, n2 * 16) != hipSuccess) return seterr(c, PN1_ENOMEM, "copy buffer");
-    if (hipMalloc(&b, n2 * 16) != hipSuccess) {
+    if (dev_malloc(&a, n2 * 16) != hipSuccess) return seterr(c, PN1_ENOMEM, "copy buffer");
+    if (dev_malloc(&b, n2 * 16) != hipSuccess) {
         (void)hipFree(a);
         return seterr(c, PN1_ENOMEM, "copy buffer");
     }

@@ -59,6 +59,30 @@ void adj_free(pn1_ctx* c);   // pn1_adj.hip
 
 }  // namespace pn1host
 
+// hipMalloc / hipHostMalloc whose failure is returned as a status only: HIP
+// also keeps a failed call's code for the next hipGetLastError(), which the
+// launch checks read, so a refused allocation would otherwise fail the next,
+// valid launch.  *p is null on failure.
+inline hipError_t dev_malloc(void** p, size_t bytes)
+{
+    const hipError_t r = hipMalloc(p, bytes);
+    if (r != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+    }
+    return r;
+}
+
+inline hipError_t host_malloc(void** p, size_t bytes)
+{
+    const hipError_t r = hipHostMalloc(p, bytes);
+    if (r != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+    }
+    return r;
+}
+
 // Opens a timed region on stream `st`: two events handed to `ev` at once
 // (so a later error leaks neither), the first recorded now; the caller
 // records *end after the launches it times.

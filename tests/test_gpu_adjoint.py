@@ -167,6 +167,33 @@ def test_adjoint_rejects_changed_state(ctx):
         ctx.adjoint_backward(np.zeros_like(x), np.zeros_like(v))
 
 
+def test_refused_allocation_leaves_context_usable():
+    """An adjoint run whose checkpoint segment cannot be allocated (10^9
+    stage states of 6 x 4097 doubles, ~590 TB) returns PN1_ENOMEM; the
+    context keeps working (the next launches do not see the refused
+    allocation's error), and a backward pass has nothing to reverse until a
+    forward run completes."""
+    x, v, m = pn.ic_plummer(4096, 26, a=1.0, m_bh=0.02)
+    c = pn.Context(len(m))
+    try:
+        c.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+        c.upload(x, v, m)
+        c.adjoint_forward(2, 2)                 # a complete run first
+        a0 = c.force()
+        with pytest.raises(pn.Pn1Error) as ei:
+            c.adjoint_forward(10**9, 10**9)
+        assert ei.value.code == pn.PN1_ENOMEM
+        assert np.array_equal(c.force(), a0)
+        with pytest.raises(pn.Pn1Error, match="no pn1_adjoint_forward"):
+            c.adjoint_backward(np.zeros_like(x), np.zeros_like(v))
+        c.adjoint_forward(2, 2)
+        gx0, gv0, gm0 = c.adjoint_backward(np.zeros_like(x), np.ones_like(v))
+        assert np.isfinite(gx0).all() and np.isfinite(gv0).all() and np.isfinite(gm0)
+        c.step(2)
+    finally:
+        c.close()
+
+
 def test_adjoint_c4_vs_finite_differences(ctx):
     """The differentiation workload: N = 16384 Plummer (seed 4) + BH 0.02, c = 100,
     eps = 1e-3, dt = 2^-10, 256 steps, checkpoints every 16, loss = mean
