@@ -32,6 +32,7 @@ namespace pn1dev {
 // window [w0, w1) of the chunk (masked self pairs).
 struct EGeom {
     int li, i, w0, w1;
+    bool live;   // the wave holds at least one of the cluster's stars
 };
 
 __device__ __forceinline__ EGeom ens_geom(const EItem& w)
@@ -39,9 +40,11 @@ __device__ __forceinline__ EGeom ens_geom(const EItem& w)
     EGeom g;
     g.li = w.ib * WG + (int)threadIdx.x;
     g.i = w.off + 1 + (g.li < w.nst ? g.li : w.nst - 1);
-    const int wb = w.off + 1 + w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
+    const int wl = w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63);
+    const int wb = w.off + 1 + wl;
     g.w0 = max(w.j0, wb);
     g.w1 = min(w.j1, wb + 64);
+    g.live = wl < w.nst;
     return g;
 }
 
@@ -60,6 +63,9 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
         const I64 I = load_i64(jr, st, P.cap, g.i, P.eps2);
+        // a wave past the cluster's last star (a partly filled last i
+        // block) skips the loop: it would only repeat that star's sums
+        if (!g.live) continue;
         double a[NACC];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0;
@@ -94,6 +100,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
         const I32 I = load_i32(jr, st, P.cap, g.i, P.eps2);
+        if (!g.live) continue;                 // as in k_ens_pair64
         float a[NACC], sN[3], cN[3];
 #pragma unroll
         for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
