@@ -391,6 +391,7 @@ __global__ __launch_bounds__(WG) void k_ens_adj_rev(const ARec* __restrict__ rec
         const int nst = w.nst;
         const int li = w.ib * WG + (int)threadIdx.x;
         const bool iok = li < nst;
+        const bool live = w.ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63) < nst;
         const int i = w.off + 1 + (iok ? li : nst - 1);
         const ABH B = load_abh(y, mu, m, P.cap, w.off);
         const RevU U = rev_uniform(B, P.eps2, P.lam);
@@ -403,7 +404,7 @@ __global__ __launch_bounds__(WG) void k_ens_adj_rev(const ARec* __restrict__ rec
         double* pj = base + (size_t)S * NADJ1 * nst;
         for (int jt = w.j0; jt < w.j1; jt += 64) {
             double g[NADJ2];
-            rev_tile(rec, I, U, i, iok, jt, w.j1, ai, g, tls[wv]);
+            rev_tile(rec, I, U, i, iok, live, jt, w.j1, ai, g, tls[wv]);
 #pragma unroll
             for (int k = 0; k < NADJ2; ++k) sh[k][wv][lane] = g[k];
             __syncthreads();

@@ -330,9 +330,13 @@ __device__ __forceinline__ void rev_next(RevJAcc& a)
 // by every thread of the workgroup (it holds a barrier); the caller's
 // barriers after the tile keep the next tile's stores behind these reads.
 constexpr int REVJ_F = 9;
+// `live` (wave-uniform) is false for a wave wholly past the last star of a
+// partly filled i block: its pairs all carry weight 0, so it skips the
+// steps and its j-side sums stay zero (it still stages and meets the
+// workgroup's barriers).
 __device__ __forceinline__ void rev_tile(const ARec* __restrict__ rec, const RevI& I,
-                                         const RevU& U, int i, bool iok, int jt, int jend,
-                                         RevIAcc& ai, double (&g)[NADJ2],
+                                         const RevU& U, int i, bool iok, bool live, int jt,
+                                         int jend, RevIAcc& ai, double (&g)[NADJ2],
                                          double (*tl)[64])
 {
     const int lane = (int)(threadIdx.x & 63);
@@ -344,16 +348,18 @@ __device__ __forceinline__ void rev_tile(const ARec* __restrict__ rec, const Rev
     __syncthreads();
     RevJAcc aj;
     rev_zero(aj);
-    for (int t = 0; t < 64; ++t) {
-        const int slot = (lane + t) & 63;
-        RevJ J;
-        J.x[0] = tl[0][slot]; J.x[1] = tl[1][slot]; J.x[2] = tl[2][slot];
-        J.v[0] = tl[3][slot]; J.v[1] = tl[4][slot]; J.v[2] = tl[5][slot];
-        J.rc2 = tl[6][slot]; J.rce = tl[7][slot]; J.m = tl[8][slot];
-        const int j = jt + slot;
-        const double Pm = (iok && j < jend && j != i) ? J.m : 0.0;
-        rev_pair(I, J, U, Pm, ai, aj);
-        rev_next(aj);
+    if (live) {
+        for (int t = 0; t < 64; ++t) {
+            const int slot = (lane + t) & 63;
+            RevJ J;
+            J.x[0] = tl[0][slot]; J.x[1] = tl[1][slot]; J.x[2] = tl[2][slot];
+            J.v[0] = tl[3][slot]; J.v[1] = tl[4][slot]; J.v[2] = tl[5][slot];
+            J.rc2 = tl[6][slot]; J.rce = tl[7][slot]; J.m = tl[8][slot];
+            const int j = jt + slot;
+            const double Pm = (iok && j < jend && j != i) ? J.m : 0.0;
+            rev_pair(I, J, U, Pm, ai, aj);
+            rev_next(aj);
+        }
     }
     rev_jfin(J0, U, aj, g);
 }
@@ -374,6 +380,7 @@ __global__ __launch_bounds__(WG) void k_adj_rev(const ARec* __restrict__ rec,
     const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
     const int li = ib * WG + (int)threadIdx.x + 1;
     const bool iok = li < n;
+    const bool live = ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63) + 1 < n;
     const int i = iok ? li : n - 1;
     const ABH B = load_abh(y, mu, m, P.cap, 0);
     const RevU U = rev_uniform(B, P.eps2, P.lam);
@@ -387,7 +394,7 @@ __global__ __launch_bounds__(WG) void k_adj_rev(const ARec* __restrict__ rec,
     __shared__ double tls[4][REVJ_F][64];
     for (int jt = j0; jt < j1; jt += 64) {
         double g[NADJ2];
-        rev_tile(rec, I, U, i, iok, jt, j1, ai, g, tls[wv]);
+        rev_tile(rec, I, U, i, iok, live, jt, j1, ai, g, tls[wv]);
 #pragma unroll
         for (int k = 0; k < NADJ2; ++k) sh[k][wv][lane] = g[k];
         __syncthreads();
