@@ -238,9 +238,12 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
             e0 = ens.energy()
             wall = time_steps(ens, steps, warmup, eager)
             e1 = ens.energy()
+            # pair-kernel time: HIP events around each persistent pair launch
+            # of 3 steps (9 launches), after one untimed step
             ens.upload(cl)                          # keeps the tuned shape
-            ens.profile_enable(True)
             ens.step(1)
+            ens.profile_enable(True)
+            ens.step(3)
             k, ms = ens.profile_read()
             ens.profile_enable(False)
             drift = np.abs((e1[:, 4] - e0[:, 4]) / np.abs(e0[:, 4]))
