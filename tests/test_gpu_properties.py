@@ -104,6 +104,33 @@ def test_rotation_covariance_gpu():
     assert rel_err(ar, a @ q.T).max() < 1e-12
 
 
+@pytest.mark.parametrize("mode", [pn.FP64, pn.MIXED])
+def test_velocity_parity_bitwise_gpu(mode):
+    """Every velocity term of force() and energy() is even in v
+    (Limit-sticky9.c:54-97, 213-422, 445-558), and negation is exact in
+    floating point: a(x, -v) and E(x, -v) equal a(x, v) and E(x, v) bit for
+    bit in both forms (the energy is FP64 in both)."""
+    x, v, m = pn.ic_plummer(4096, 13, a=1.0, m_bh=0.02)
+    with pn.Context(len(m)) as ctx:
+        a = force_at(ctx, x, v, m, mode, 1e-3, 10.0)
+        e = ctx.energy()
+        an = force_at(ctx, x, -v, m, mode, 1e-3, 10.0)
+        en = ctx.energy()
+    assert np.array_equal(a, an)
+    assert np.array_equal(e, en)
+
+
+def test_translation_invariance_gpu():
+    """Shifting every position, the BH's included, leaves force() unchanged
+    up to the rounding of the shifted coordinates (FP64 form)."""
+    x, v, m = pn.ic_uniform_sphere(4096, 14, R=1.0, Q=0.5, m_bh=0.01, eps=1e-3)
+    shift = np.array([0.375, -1.25, 2.5])             # exact in binary
+    with pn.Context(len(m)) as ctx:
+        a = force_at(ctx, x, v, m, pn.FP64, 1e-3, 100.0)
+        at = force_at(ctx, x + shift, v, m, pn.FP64, 1e-3, 100.0)
+    assert rel_err(at, a).max() < 1e-11
+
+
 def test_energy_vs_oracle_n4097():
     x, v, m = pn.ic_plummer(4096, 12, a=1.0, m_bh=0.02)
     with pn.Context(len(m)) as ctx:
