@@ -131,6 +131,21 @@ def test_translation_invariance_gpu():
     assert rel_err(at, a).max() < 1e-11
 
 
+def test_star_permutation_gpu():
+    """Relabelling the stars (the BH stays first) permutes the accelerations
+    and leaves the energy unchanged, up to the reordered sums (FP64 form)."""
+    x, v, m = pn.ic_plummer(4096, 15, a=1.0, m_bh=0.02)
+    m = m * np.random.default_rng(5).uniform(0.5, 1.5, len(m))
+    p = np.concatenate([[0], 1 + np.random.default_rng(6).permutation(len(m) - 1)])
+    with pn.Context(len(m)) as ctx:
+        a = force_at(ctx, x, v, m, pn.FP64, 1e-3, 100.0)
+        e = ctx.energy()
+        ap = force_at(ctx, x[p], v[p], m[p], pn.FP64, 1e-3, 100.0)
+        ep = ctx.energy()
+    assert rel_err(ap, a[p]).max() < 1e-12
+    assert np.all(np.abs(ep - e) <= 1e-12 * np.abs(e).max())
+
+
 def test_energy_vs_oracle_n4097():
     x, v, m = pn.ic_plummer(4096, 12, a=1.0, m_bh=0.02)
     with pn.Context(len(m)) as ctx:
