@@ -182,13 +182,13 @@ void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, 
     if (mixed) {
         float* pp = (float*)c->part;
         switch (variant) {
-        case 0: hipLaunchKernelGGL(k_pair32<0>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+        case 0: hipLaunchKernelGGL(k_pair32<0>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
                                    c->tile); break;
-        case 1: hipLaunchKernelGGL(k_pair32<1>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+        case 1: hipLaunchKernelGGL(k_pair32<1>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
                                    c->tile); break;
-        case 2: hipLaunchKernelGGL(k_pair32<2>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+        case 2: hipLaunchKernelGGL(k_pair32<2>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
                                    c->tile); break;
-        default: hipLaunchKernelGGL(k_pair32<3>, g, blk, 0, c->st, c->j32, in, pp, P, S, nib, ib0,
+        default: hipLaunchKernelGGL(k_pair32<3>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
                                     c->tile); break;
         }
     } else if (variant & 1) {

@@ -29,14 +29,18 @@ constexpr int NEB = 4;      // energy per-block partials
 //          and added by k_fin as al_i N
 //   U    = sum K_u s_j,  KS = sum K_u      (sum K_u u = U / sqrt(3/2) - v_i KS)
 //   T3   = sum 3.5 m0 m_j e_j^3 w d_j      (k_fin subtracts e_i times its w-free sum)
-//   phi  = sum m_j w,  xi = sum m_j w^3 (D2_i - D2_j),  zeta = sum m_j 0.25 m0 e_j^3 t
+//   phi  = sum m_j w,  W3 = sum m_j w^3,  zeta = sum m_j 0.25 m0 e_j^3 t
+//          (the BH's a1c needs xi = sum m_j w^3 (D2_i - D2_j); with d_j = d_i + r
+//          and w^3 |r|^2 = w - eps^2 w^3 it is -2 d_i.N - phi + eps^2 W3, which
+//          k_fin forms in FP64: one add per pair instead of a subtract and an FMA,
+//          and no |d_j|^2 in the j-record)
 // zeta shares t with K_r, as the reference's two e_j^3 terms share
 // r1b2 - rbc2 (Limit-sticky9.c:302, 305); the two cancel strongly when star
 // j is close to the BH, and the shared rounding cancels with them.  The
 // (w - e_i) c_j term (:307-308) is summed over d_j = x_j - x_BH, never over
 // r + d_i, which would cancel catastrophically in the same situation.
 enum { A_NX, A_NY, A_NZ, A_RX, A_RY, A_RZ, A_UX, A_UY, A_UZ, A_TX, A_TY, A_TZ,
-       A_KS, A_PHI, A_XI, A_ZE };
+       A_KS, A_PHI, A_W3, A_ZE };
 // energy pair accumulators: phi, sum m_j w s_j, sum m_j w^3 (r.v_i)(r.s_j)
 enum { E_PHI, E_PVX, E_PVY, E_PVZ, E_GAM };
 // separable sums over stars: sum m e, sum m e^3 d, sum m v
@@ -53,12 +57,12 @@ struct __align__(16) JRec64 {
 };
 // j-record, mixed form: double-single position (hi, lo), the rest rounded
 // to FP32, and Qd = Q d (the pair term Q w d is then one FMA per component).
-// 80 bytes.
+// 64 bytes: one 16-dword scalar load.
 struct __align__(16) JRec32 {
-    float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, D2, A, B, MB, Qdx, Qdy, Qdz, pad[3];
+    float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, A, B, MB, Qdx, Qdy, Qdz;
 };
 static_assert(sizeof(JRec64) == 128, "JRec64 layout");
-static_assert(sizeof(JRec32) == 80, "JRec32 layout");
+static_assert(sizeof(JRec32) == 64, "JRec32 layout");
 
 // Everything a kernel needs about the run, passed by value.
 struct Phys {

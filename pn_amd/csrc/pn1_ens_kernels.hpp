@@ -86,6 +86,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
 
 template <int PP>
 __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr,
+                                                  const JRec64* __restrict__ j64,
                                                   const double* __restrict__ st,
                                                   float* __restrict__ part,
                                                   const EItem* __restrict__ items,
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
         if (it < 0) break;
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
-        const I32 I = load_i32(jr, st, P.cap, g.i, P.eps2);
+        const I32 I = load_i32(jr, j64, st, P.cap, g.i, P.eps2);
         if (!g.live) continue;                 // as in k_ens_pair64
         float a[NACC], sN[3], cN[3];
 #pragma unroll

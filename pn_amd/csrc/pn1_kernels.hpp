@@ -97,10 +97,9 @@ __device__ __forceinline__ void prep_star(const double* __restrict__ st,
         h.yl = (float)(y - (double)h.yh);
         h.zl = (float)(z - (double)h.zh);
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
-        h.m = (float)mp; h.D2 = (float)D2;
+        h.m = (float)mp;
         h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
         h.Qdx = (float)(r.Q * dx); h.Qdy = (float)(r.Q * dy); h.Qdz = (float)(r.Q * dz);
-        h.pad[0] = h.pad[1] = h.pad[2] = 0.0f;
         j32[p] = h;
     }
     g[G_G1] = mp * e;
@@ -142,7 +141,7 @@ __global__ __launch_bounds__(WG) void k_rows_sum(const double* __restrict__ rows
 // Per-lane state of the i particle: position, g = 4 v, h = 4 v / sqrt(3/2),
 // |d|^2 and |d|^2 + eps^2.
 struct I64 {
-    double x, y, z, gx, gy, gz, hx, hy, hz, D2, D2e;
+    double x, y, z, gx, gy, gz, hx, hy, hz, D2e;
 };
 
 __device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr,
@@ -155,7 +154,6 @@ __device__ __forceinline__ I64 load_i64(const JRec64* __restrict__ jr,
     I.x = r.x; I.y = r.y; I.z = r.z;
     I.gx = 4.0 * vx; I.gy = 4.0 * vy; I.gz = 4.0 * vz;
     I.hx = PN1_4OSQ15 * vx; I.hy = PN1_4OSQ15 * vy; I.hz = PN1_4OSQ15 * vz;
-    I.D2 = r.D2;
     I.D2e = r.D2 + eps2;
     return I;
 }
@@ -181,7 +179,7 @@ __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64&
     a[A_NY] = __builtin_fma(mw3, ry, a[A_NY]);
     a[A_NZ] = __builtin_fma(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
-    a[A_XI] = __builtin_fma(mw3, I.D2 - J.D2, a[A_XI]);
+    a[A_W3] += mw3;
     const double qw = Qj * w;
     a[A_TX] = __builtin_fma(qw, J.dx, a[A_TX]);
     a[A_TY] = __builtin_fma(qw, J.dy, a[A_TY]);
@@ -269,23 +267,25 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
 // positions, FP32 tile sums folded into Kahan-compensated FP32 totals
 // ---------------------------------------------------------------------
 struct I32 {
-    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2, D2e;
+    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2e;
 };
 
+// Star i: its mixed j-record, velocity and |d_i|^2 (from its FP64 record).
 __device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
+                                        const JRec64* __restrict__ j64,
                                         const double* __restrict__ st, int cap, int i,
                                         double eps2)
 {
     const JRec32& r = jr[i];
     const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
+    const double D2 = j64[i].D2;
     I32 I;
     I.xh = r.xh; I.yh = r.yh; I.zh = r.zh;
     I.xl = r.xl; I.yl = r.yl; I.zl = r.zl;
     I.gx = (float)(4.0 * vx); I.gy = (float)(4.0 * vy); I.gz = (float)(4.0 * vz);
     I.hx = (float)(PN1_4OSQ15 * vx); I.hy = (float)(PN1_4OSQ15 * vy);
     I.hz = (float)(PN1_4OSQ15 * vz);
-    I.D2 = r.D2;
-    I.D2e = (float)((double)r.D2 + eps2);
+    I.D2e = (float)(D2 + eps2);
     return I;
 }
 
@@ -320,7 +320,7 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     a[A_NY] = __builtin_fmaf(mw3, ry, a[A_NY]);
     a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
-    a[A_XI] = __builtin_fmaf(mw3, I.D2 - J.D2, a[A_XI]);
+    a[A_W3] += mw3;
     a[A_TX] = __builtin_fmaf(w, Qdx, a[A_TX]);
     a[A_TY] = __builtin_fmaf(w, Qdy, a[A_TY]);
     a[A_TZ] = __builtin_fmaf(w, Qdz, a[A_TZ]);
@@ -407,6 +407,7 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int
 // and accumulate in plain FP32 over the whole chunk.
 template <int PP>
 __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
+                                              const JRec64* __restrict__ j64,
                                               const double* __restrict__ st,
                                               float* __restrict__ part, Phys P, int S,
                                               int nIblk, int ib0, int tile)
@@ -418,7 +419,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(jr, st, P.cap, i, P.eps2);
+    const I32 I = load_i32(jr, j64, st, P.cap, i, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
     float a[NACC], sumN[3], cmpN[3];
@@ -545,7 +546,10 @@ __device__ __forceinline__ void fin_star(const double* in, double* X, double* Y,
     const double mE3 = mp * E3;
     const double cA1 = mE3 * (5.0 * m0 * e - 2.0 * V2 + 1.5 * vd * vd / D2);
     const double cC0 = 4.0 * mp * mp * e * e * e * e;
-    const double cC2 = mE3 * (4.0 * (G1 - mp * e) + 1.25 * S_[A_PHI] + 0.25 * S_[A_XI]
+    // xi_i = sum' m_j w^3 (D2_i - D2_j) = -2 d_i.N - phi + eps^2 W3
+    const double xi = -2.0 * (d[0] * S_[A_NX] + d[1] * S_[A_NY] + d[2] * S_[A_NZ])
+                      - S_[A_PHI] + eps2 * S_[A_W3];
+    const double cC2 = mE3 * (4.0 * (G1 - mp * e) + 1.25 * S_[A_PHI] + 0.25 * xi
                               + mp * self_br);
     const double cQ2 = mp * mp * E3 * (4.0 * e + 1.25 * C.eps);
     const double vm = mE3 / m0, vm2 = mp * mp * E3 / m0;

@@ -20,7 +20,8 @@ Pair sums (per star i), the only O(n^2) work:
   RU, KS = sum' K_u v_j, sum' K_u,  K_u = m_j w^3 (r.(4 v_i - 3 v_j)),
           so that sum' K_u u = RU - v_i KS
   T3    = sum' 3.5 m0 q_j w d_j   (the O(n) pass subtracts e_i sum' 3.5 m0 q_j d_j)
-  phi   = sum' m_j w,  Xi = sum' m_j w^3 (D2_i - D2_j)
+  phi   = sum' m_j w,  W3 = sum' m_j w^3, and from them (xi_from_sums)
+  Xi    = sum' m_j w^3 (D2_i - D2_j) = -2 d_i.Nst - phi + eps^2 W3
   zeta  = sum' m_j 0.25 m0 e_j^3 (|r|^2 - D2_i)
 The (w - e_i) d_j term (Limit-sticky9.c:307-308) is summed over d_j, the
 star-BH separation: re-expressing d_j as r + d_i instead would turn it
@@ -75,9 +76,18 @@ def pair_sums(x, v, m, eps):
         KS=Ku.sum(1),
         T3=qw @ d,
         phi=mw.sum(1),
-        Xi=(mw3 * (D2[:, None] - D2[None, :])).sum(1),
+        W3=mw3.sum(1),
+        Xi_direct=(mw3 * (D2[:, None] - D2[None, :])).sum(1),
         zeta=np.where(live, (0.25 * m0 * q)[None, :] * t, 0.0).sum(1),
     )
+
+
+def xi_from_sums(d, Nst, phi, W3, eps):
+    """Xi_i = sum' m_j w^3 (D2_i - D2_j) from sums the pair loop has anyway.
+    With d_j = d_i + r, D2_i - D2_j = -2 d_i.r - |r|^2, and w^3 |r|^2 =
+    w - eps^2 w^3 (w = (|r|^2 + eps^2)^-1/2), so
+        Xi_i = -2 d_i.Nst_i - phi_i + eps^2 W3_i,  W3_i = sum' m_j w^3."""
+    return -2.0 * np.einsum("ij,ij->i", d, Nst) - phi + eps * eps * W3
 
 
 def force(x, v, m, eps, c):
@@ -94,6 +104,7 @@ def force(x, v, m, eps, c):
     M1 = (q[:, None] * d).sum(0)
     P = (ms[:, None] * vs).sum(0)
     S = pair_sums(x, v, m, eps)
+    Xi = xi_from_sums(d, S["Nst"], S["phi"], S["W3"], eps)
 
     a = np.zeros_like(x)
     # stars
@@ -127,7 +138,7 @@ def force(x, v, m, eps, c):
     ws = 1.0 / np.sqrt(eps * eps)
     eps2 = eps * eps
     self_br = ((4.0 * e + 1.25 * ws) - 0.25 * D2 / eps2 * ws) + 0.25 * D2 / eps2 * ws
-    C2 = (ms * E3)[:, None] * d * (4 * (G1 - ms * e) + 1.25 * S["phi"] + 0.25 * S["Xi"]
+    C2 = (ms * E3)[:, None] * d * (4 * (G1 - ms * e) + 1.25 * S["phi"] + 0.25 * Xi
                                    + ms * self_br)[:, None] \
         - (3.5 * e * ms)[:, None] * S["Nst"] \
         - (ms * E3 / m0)[:, None] * (4 * vP0[:, None] * d - 3 * vd[:, None] * P[None, :]

@@ -167,6 +167,31 @@ def test_decomposition_matches_oracle(n, c):
     assert np.all(np.abs(eh - e) <= 4e-15 * scale + 1e-13 * np.abs(e).max())
 
 
+@pytest.mark.parametrize("seed,eps", [(40, 1e-3), (41, 1e-4), (42, 0.1)])
+def test_xi_identity(seed, eps):
+    """The BH's a1c needs Xi_i = sum' m_j w^3 (D2_i - D2_j); the kernels
+    accumulate only W3 = sum' m_j w^3 and rebuild Xi as -2 d_i.N - phi +
+    eps^2 W3.  The two agree to rounding, in 80-bit arithmetic on a system
+    with stars far out (|d| ~ 30), one next to the BH and one close pair."""
+    x, v, m, _ = rand_state(65, seed, eps)
+    x[1] = x[0] + [1e-3, 0.0, 0.0]
+    x[2] = [30.0, -4.0, 2.0]
+    x[3] = [-25.0, 11.0, 7.0]
+    x[5] = x[4] + [0.0, 2e-4, 0.0]
+    L = np.longdouble
+    xl, vl, ml = x.astype(L), v.astype(L), m.astype(L)
+    S = hm.pair_sums(xl, vl, ml, L(eps))
+    d, D2, e = hm.prep(xl, vl, ml, L(eps))
+    xi = hm.xi_from_sums(d, S["Nst"], S["phi"], S["W3"], L(eps))
+    # scale: the magnitudes that cancel on either side
+    scale = (2 * np.linalg.norm(d, axis=1) * np.linalg.norm(S["Nst"], axis=1)
+             + S["phi"] + np.abs(S["Xi_direct"]))
+    assert (np.abs(xi - S["Xi_direct"]) / scale).astype(float).max() < 1e-16
+    # and a planted mistake (the factor 2 dropped) fails it
+    bad = -np.einsum("ij,ij->i", d, S["Nst"]) - S["phi"] + L(eps) ** 2 * S["W3"]
+    assert (np.abs(bad - S["Xi_direct"]) / scale).astype(float).max() > 1e-3
+
+
 def test_oracle_against_extended_precision():
     """The oracle (FP64, reference operation order) against the same
     formulas in 80-bit arithmetic: at c = 50 its stars are good to 1e-13."""
