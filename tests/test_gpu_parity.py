@@ -19,6 +19,7 @@ Tolerances:
 import numpy as np
 import pytest
 
+import hoisted_model as hm
 import oracle_bind as ob
 from conftest import FORCE_FIXTURES, c_values, energy_atol, load_fixture, rel_err
 from pn_amd import lib as pn
@@ -140,3 +141,47 @@ def test_mixed_vs_fp64(ctx, name):
         assert np.quantile(stars, 0.999) <= (5e-6 if c == 1 else 1e-6)
         assert stars.max() <= 1e-5
         assert err[0] <= 1e-6
+
+
+@pytest.mark.parametrize("c", [1.0, 50.0])
+def test_bh_row_far_and_near_stars(ctx, c):
+    """The BH's a1c bracket takes xi_i = sum' m_j w^3 (D2_i - D2_j), which
+    the library rebuilds from its pair sums as -2 d_i.N - phi + eps^2 W3.
+    The cancellation in that form is largest for stars far from the BH:
+    n = 257 with stars at |d| ~ 20-40, one 1e-3 from the BH and a close
+    pair.  FP64: star rows within 1e-12 of the oracle at c = 50 and within
+    the reference's own rounding at c = 1 (checked against an 80-bit
+    evaluation), the BH's within the reference's a1c rounding; mixed: the
+    BH within 1e-6 of FP64."""
+    rng = np.random.default_rng(77)
+    n = 257
+    x = rng.uniform(-1, 1, (n, 3))
+    v = rng.normal(0, 0.3, (n, 3))
+    m = rng.uniform(0.5, 1.5, n) / (n - 1)
+    m[0] = 0.05
+    x[0] = 0.0
+    v[0] = 0.0
+    far = rng.normal(size=(12, 3))
+    far *= (rng.uniform(20, 40, 12) / np.linalg.norm(far, axis=1))[:, None]
+    x[1:13] = far
+    x[13] = [1e-3, 0.0, 0.0]
+    x[15] = x[14] + [0.0, 2e-4, 0.0]
+    eps = 1e-3
+    d = dict(x=x, v=v, m=m, eps=eps)
+    a64 = gpu_force(ctx, d, c, pn.FP64)
+    ref = ob.orc_force(x, v, m, eps, c)
+    err = rel_err(a64, ref)
+    # the same formulas in 80-bit arithmetic (tests/hoisted_model.py)
+    L = np.longdouble
+    ex = hm.force(x.astype(L), v.astype(L), m.astype(L), L(eps), L(c))
+    e80 = rel_err(a64.astype(L), ex).astype(float)
+    o80 = rel_err(ref.astype(L), ex).astype(float)
+    print(f"far/near c={c}: vs oracle star max {err[1:].max():.3e} BH {err[0]:.3e}; "
+          f"vs 80-bit GPU {e80.max():.3e}, oracle {o80.max():.3e}")
+    # At c = 1 the far stars' 1PN terms cancel against each other, and the
+    # reference's own order is 6.2e-12 from the 80-bit value there.
+    assert err[1:].max() <= (2e-11 if c == 1.0 else 1e-12)
+    assert e80[1:].max() <= 2e-11
+    assert np.abs(a64[0] - ref[0]).max() <= max(1e-12 * np.abs(ref[0]).max(), bh_a1c_noise(d, c))
+    a32 = gpu_force(ctx, d, c, pn.MIXED)
+    assert rel_err(a32, a64)[0] <= 1e-6
