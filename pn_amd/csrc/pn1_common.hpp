@@ -50,10 +50,10 @@ enum { G_G1, G_M1X, G_M1Y, G_M1Z, G_PX, G_PY, G_PZ };
 //   x            position                 s = sqrt(3/2) v
 //   m, D2        mass, |d|^2, d = x - x_BH
 //   A            1.25 m0 e - 2 |v|^2       B = 0.25 m0 e^3
-//   Q            3.5 m0 m e^3              MB = m B
-//   d            x - x_BH                  e = (|d|^2 + eps^2)^-1/2
+//   MB = m B     Qd = Q d, Q = 3.5 m0 m e^3 (the pair term Q w d is then one FMA
+//                per component)            e = (|d|^2 + eps^2)^-1/2
 struct __align__(16) JRec64 {
-    double x, y, z, sx, sy, sz, m, D2, A, B, Q, MB, dx, dy, dz, e;
+    double x, y, z, sx, sy, sz, m, D2, A, B, MB, Qdx, Qdy, Qdz, e, pad;
 };
 // j-record, mixed form: double-single position (hi, lo), the rest rounded
 // to FP32, and Qd = Q d (the pair term Q w d is then one FMA per component).

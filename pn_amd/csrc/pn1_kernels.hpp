@@ -87,8 +87,10 @@ __device__ __forceinline__ void prep_star(const double* __restrict__ st,
     r.sx = PN1_SQ15 * vx; r.sy = PN1_SQ15 * vy; r.sz = PN1_SQ15 * vz;
     r.m = mp; r.D2 = D2;
     r.A = 1.25 * m0 * e - 2.0 * V2; r.B = 0.25 * m0 * E3;
-    r.Q = 3.5 * m0 * q; r.MB = mp * r.B;
-    r.dx = dx; r.dy = dy; r.dz = dz; r.e = e;
+    const double Q = 3.5 * m0 * q;
+    r.MB = mp * r.B;
+    r.Qdx = Q * dx; r.Qdy = Q * dy; r.Qdz = Q * dz;
+    r.e = e; r.pad = 0.0;
     j64[p] = r;
     if (j32) {
         JRec32 h;
@@ -99,7 +101,7 @@ __device__ __forceinline__ void prep_star(const double* __restrict__ st,
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
         h.m = (float)mp;
         h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
-        h.Qdx = (float)(r.Q * dx); h.Qdy = (float)(r.Q * dy); h.Qdz = (float)(r.Q * dz);
+        h.Qdx = (float)r.Qdx; h.Qdy = (float)r.Qdy; h.Qdz = (float)r.Qdz;
         j32[p] = h;
     }
     g[G_G1] = mp * e;
@@ -162,12 +164,14 @@ template <bool MASK>
 __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64& I,
                                        double (&a)[NACC], double eps2)
 {
-    double mj = J.m, Qj = J.Q, MBj = J.MB;
+    double mj = J.m, MBj = J.MB, Qdx = J.Qdx, Qdy = J.Qdy, Qdz = J.Qdz;
     if (MASK) {
         const bool self = (j == i);
         mj = self ? 0.0 : mj;
-        Qj = self ? 0.0 : Qj;
         MBj = self ? 0.0 : MBj;
+        Qdx = self ? 0.0 : Qdx;
+        Qdy = self ? 0.0 : Qdy;
+        Qdz = self ? 0.0 : Qdz;
     }
     const double rx = J.x - I.x, ry = J.y - I.y, rz = J.z - I.z;
     const double R2e = __builtin_fma(rx, rx, __builtin_fma(ry, ry, __builtin_fma(rz, rz, eps2)));
@@ -180,10 +184,9 @@ __device__ __forceinline__ void body64(const JRec64& J, int j, int i, const I64&
     a[A_NZ] = __builtin_fma(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
     a[A_W3] += mw3;
-    const double qw = Qj * w;
-    a[A_TX] = __builtin_fma(qw, J.dx, a[A_TX]);
-    a[A_TY] = __builtin_fma(qw, J.dy, a[A_TY]);
-    a[A_TZ] = __builtin_fma(qw, J.dz, a[A_TZ]);
+    a[A_TX] = __builtin_fma(w, Qdx, a[A_TX]);
+    a[A_TY] = __builtin_fma(w, Qdy, a[A_TY]);
+    a[A_TZ] = __builtin_fma(w, Qdz, a[A_TZ]);
     const double t = R2e - I.D2e;
     a[A_ZE] = __builtin_fma(MBj, t, a[A_ZE]);
     const double sr = __builtin_fma(J.sx, rx, __builtin_fma(J.sy, ry, J.sz * rz));
