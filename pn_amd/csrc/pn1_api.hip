@@ -67,6 +67,7 @@ Phys make_phys(const pn1_ctx* c)
     P.dt = c->dt;
     P.m0 = c->m0;
     P.ws = 1.0 / std::sqrt(P.eps2);
+    P.mmax = c->mmax;
     P.eps2f = (float)P.eps2;
     P.m0f = (float)c->m0;
     P.n = c->n;
@@ -108,7 +109,7 @@ int ensure_part(pn1_ctx* c, int split = 0)
     const size_t cap = (size_t)c->cap, S = (size_t)(split > 0 ? split : c->S);
     size_t need = S * NACC * cap * sizeof(double);
     const size_t needE = S * NACCE * cap * sizeof(double);
-    const size_t need32 = S * (NACC + 3) * cap * sizeof(float);
+    const size_t need32 = S * MROWS * cap * sizeof(float);
     if (needE > need) need = needE;
     if (need32 > need) need = need32;
     if (need <= c->part_bytes) return PN1_OK;
@@ -161,8 +162,8 @@ template <int MODE, int STAGE>
 void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
 {
     hipLaunchKernelGGL((k_fin<MODE, STAGE>), dim3(c->nblk), dim3(WG), 0, c->st, in, c->X, c->Y,
-                       c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk, P,
-                       1, c->n);
+                       c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk,
+                       c->kr, P, 1, c->n);
     hipLaunchKernelGGL((k_bh<STAGE>), dim3(1), dim3(WG), 0, c->st, c->bhblk, c->nblk, c->gs, c->X,
                        c->Y, c->H1, c->H2, c->acc, P);
 }
@@ -182,13 +183,13 @@ void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, 
     if (mixed) {
         float* pp = (float*)c->part;
         switch (variant) {
-        case 0: hipLaunchKernelGGL(k_pair32<0>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
+        case 0: hipLaunchKernelGGL(k_pair32<0>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
                                    c->tile); break;
-        case 1: hipLaunchKernelGGL(k_pair32<1>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
+        case 1: hipLaunchKernelGGL(k_pair32<1>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
                                    c->tile); break;
-        case 2: hipLaunchKernelGGL(k_pair32<2>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
+        case 2: hipLaunchKernelGGL(k_pair32<2>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
                                    c->tile); break;
-        default: hipLaunchKernelGGL(k_pair32<3>, g, blk, 0, c->st, c->j32, c->j64, in, pp, P, S, nib, ib0,
+        default: hipLaunchKernelGGL(k_pair32<3>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
                                     c->tile); break;
         }
     } else if (variant & 1) {
@@ -206,7 +207,9 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
     const double* in = stage <= 1 ? c->X : c->Y;
     const bool mixed = c->mode == PN1_MIXED;
     hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
-                       mixed ? c->j32 : nullptr, c->gblk);
+                       mixed ? c->j32 : nullptr, c->gblk,
+                       mixed && c->acc_ok ? (const double*)c->acc : nullptr,
+                       mixed && c->acc_ok ? (const float*)c->kr : nullptr, c->thr);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     if (c->n > 1) {
         hipEvent_t e1 = nullptr;
@@ -225,7 +228,14 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
     default: launch_fin<1, 3>(c, in, P); break;
     }
     HIPCHK(c, hipGetLastError());
+    c->acc_ok = mixed;
     return PN1_OK;
+}
+
+int ensure_prev_acc(pn1_ctx* c, const Phys& P)
+{
+    if (c->mode != PN1_MIXED || c->acc_ok) return PN1_OK;
+    return enqueue_force(c, 0, P);
 }
 
 int enqueue_step(pn1_ctx* c, const Phys& P)
@@ -278,6 +288,8 @@ void pn1_destroy(pn1_ctx* c)
         if (b) (void)hipFree(b);
     if (c->j64) (void)hipFree(c->j64);
     if (c->j32) (void)hipFree(c->j32);
+    if (c->thr) (void)hipFree(c->thr);
+    if (c->kr) (void)hipFree(c->kr);
     if (c->part) (void)hipFree(c->part);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -324,6 +336,8 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
     // two records more than particles: the pair loops fetch up to jr[j + 2]
     if (!rc) rc = dalloc(c, &c->j64, cap + 2);
     if (!rc) rc = dalloc(c, &c->j32, cap + 2);
+    if (!rc) rc = dalloc(c, &c->thr, cap);
+    if (!rc) rc = dalloc(c, &c->kr, cap);
     if (!rc && host_malloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
         c->h_stage = nullptr;
         rc = PN1_ENOMEM;
@@ -405,6 +419,9 @@ int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, in
     ++c->gen;
     c->n = n;
     c->m0 = m[0];
+    c->mmax = 0.0;
+    for (int i = 1; i < n; ++i) c->mmax = std::max(c->mmax, m[i]);
+    c->acc_ok = false;
     c->have_state = true;
     plan(c);
     return ensure_part(c);
@@ -425,7 +442,9 @@ int pn1_force(pn1_ctx* c, double* a)
     int rc;
     if (!ready(c, &rc)) return rc;
     if (!a) return seterr(c, PN1_EINVAL, "null output");
-    if ((rc = enqueue_force(c, 0, make_phys(c)))) return rc;
+    const Phys P = make_phys(c);
+    if ((rc = ensure_prev_acc(c, P))) return rc;
+    if ((rc = enqueue_force(c, 0, P))) return rc;
     return fetch3(c, c->acc, a);
 }
 
@@ -453,11 +472,14 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
         const int nst = n - 1;
         while (S < 512 && (long)S * nib < 16384 && (nst + 2 * S - 1) / (2 * S) >= 64) S *= 2;
     }
+    if ((rc = ensure_prev_acc(c, P))) return rc;
     if ((rc = ensure_part(c, S))) return rc;
     // every star's j-record and the separable sums: what a sharded run would
     // all-gather and all-reduce before its pair kernel
     hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
-                       mixed ? c->j32 : nullptr, c->gblk);
+                       mixed ? c->j32 : nullptr, c->gblk,
+                       mixed ? (const double*)c->acc : nullptr,
+                       mixed ? (const float*)c->kr : nullptr, c->thr);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     hipEvent_t e1 = nullptr;
     if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
@@ -466,10 +488,13 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     const dim3 gf(nib);
     if (mixed)
         hipLaunchKernelGGL((k_fin<1, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, P, i0, i1);
+                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr, P, i0,
+                           i1);
     else
         hipLaunchKernelGGL((k_fin<0, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, P, i0, i1);
+                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr, P, i0,
+                           i1);
+    if (!mixed) c->acc_ok = false;
     HIPCHK(c, hipGetLastError());
     const size_t cap = (size_t)c->cap, cnt = (size_t)(i1 - i0);
     for (int k = 0; k < 3; ++k)
@@ -488,7 +513,8 @@ int pn1_energy(pn1_ctx* c, pn1_energy_t* out)
     if (!out) return seterr(c, PN1_EINVAL, "null output");
     const Phys P = make_phys(c);
     hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, c->X, c->m, P, c->j64,
-                       (JRec32*)nullptr, c->gblk);
+                       (JRec32*)nullptr, c->gblk, (const double*)nullptr, (const float*)nullptr,
+                       (Thr*)nullptr);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     if (c->n > 1)
         hipLaunchKernelGGL(k_epair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64, c->X,
@@ -516,6 +542,7 @@ int pn1_step(pn1_ctx* c, int steps)
     if (steps < 0) return seterr(c, PN1_EINVAL, "steps < 0");
     if (steps == 0) return PN1_OK;
     const Phys P = make_phys(c);
+    if ((rc = ensure_prev_acc(c, P))) return rc;
     if (c->prof) {
         for (int s = 0; s < steps; ++s)
             if ((rc = enqueue_step(c, P))) return rc;

@@ -41,6 +41,14 @@ constexpr int NEB = 4;      // energy per-block partials
 // r + d_i, which would cancel catastrophically in the same situation.
 enum { A_NX, A_NY, A_NZ, A_RX, A_RY, A_RZ, A_UX, A_UY, A_UZ, A_TX, A_TY, A_TZ,
        A_KS, A_PHI, A_W3, A_ZE };
+// The mixed form keeps one more sum, A_TS = sum m_j w^2 (>= sum of the
+// Newtonian terms' magnitudes, which bounds how far the FP32 rounding of
+// those terms can reach into a star's acceleration), and writes per chunk
+// and star MROWS floats: the NACC32 sums, then the three Kahan
+// compensations of the Newtonian sums.
+constexpr int A_TS = NACC;
+constexpr int NACC32 = NACC + 1;
+constexpr int MROWS = NACC32 + 3;
 // energy pair accumulators: phi, sum m_j w s_j, sum m_j w^3 (r.v_i)(r.s_j)
 enum { E_PHI, E_PVX, E_PVY, E_PVZ, E_GAM };
 // separable sums over stars: sum m e, sum m e^3 d, sum m v
@@ -57,6 +65,10 @@ struct __align__(16) JRec64 {
 };
 // j-record, mixed form: double-single position (hi, lo), the rest rounded
 // to FP32, and Qd = Q d (the pair term Q w d is then one FMA per component).
+// m is the star's mass divided by the cluster's largest star mass (mscale):
+// 1 exactly when the masses are equal, so the masses carry no rounding that
+// every FP32 pair term would share; the sums built from it (Newton, Rr, U,
+// KS, phi, W3, A_TS) are scaled back by mscale in FP64.
 // 64 bytes: one 16-dword scalar load.
 struct __align__(16) JRec32 {
     float xh, yh, zh, xl, yl, zl, sx, sy, sz, m, A, B, MB, Qdx, Qdy, Qdz;
@@ -68,8 +80,37 @@ static_assert(sizeof(JRec32) == 64, "JRec32 layout");
 struct Phys {
     double eps, eps2, lam, dt, m0;
     double ws;       // 1/sqrt(eps^2): the reference's self-pair w
+    double mmax;     // largest star mass (mixed form's pair thresholds)
     float eps2f, m0f;
     int n, cap;      // particles incl. the BH; SoA stride
+};
+
+// Mixed form: per-star pair thresholds, from the star's previous
+// acceleration |a| (the last evaluation of the context), compared with the
+// FP32 R2e = |r|^2 + eps^2 of each pair.
+//   rc2  a pair with R2e < rc2 can pull harder than PN1_CLOSE_KAPPA |a|
+//        (m_j / R2e > kappa |a|).  Its Newtonian term is evaluated in FP64
+//        and added to the compensated sum: a neighbour whose pull is several
+//        times the star's net acceleration would otherwise bring the FP32
+//        rounding of its one term (~3 u) into the result at that multiple.
+//   dn2  below it the separation needs the lo parts of the double-single
+//        positions.  With the hi parts alone each position is off by up to
+//        u |x| (u = 2^-24), which moves a pair's Newtonian term by up to
+//        6 u |x| m_j / |r|^3; dn2 is where that reaches PN1_LAZY_ETA |a|
+//        (|r|^3 = 6 u |x| mmax / (eta |a|)).  Farther pairs keep the hi parts
+//        only: their position errors act as a displacement of the star by
+//        ~u |x| in the smooth far field, and a few u of that field.
+// A star whose previous evaluation had mscale * A_TS > PN1_HARD_K |a| (its
+// pair terms cancel to 1/30 of their magnitudes or less) takes every pair in
+// FP64 (rc2 = FLT_MAX): the FP32 rounding of the terms, ~0.1 u of their
+// magnitudes once averaged, would otherwise near 1e-6 of |a|.
+// Without a previous evaluation (after an upload) the thresholds are
+// rc2 = 0 and dn2 = FLT_MAX: no FP64 pairs, lo parts everywhere.
+#define PN1_CLOSE_KAPPA 0.25
+#define PN1_LAZY_ETA 1e-8
+#define PN1_HARD_K 30.0
+struct __align__(8) Thr {
+    float rc2, dn2;
 };
 
 // ---------------------------------------------------------------------

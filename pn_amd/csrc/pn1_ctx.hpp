@@ -20,6 +20,7 @@ struct pn1_ctx {
     double eps = 1e-3, c = 1.0, dt = 1.0 / 1024.0;
     int mode = PN1_FP64;
     double m0 = 0.0;            // host copy of m[0]
+    double mmax = 0.0;          // largest star mass (mixed form's pair thresholds)
     int jsplit_req = 0, tile_req = 0, variant_req = -1;
     // launch plan for the current n
     int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 16;
@@ -29,6 +30,12 @@ struct pn1_ctx {
     double *bhblk = nullptr, *eblk = nullptr, *eout = nullptr;
     pn1dev::JRec64* j64 = nullptr;
     pn1dev::JRec32* j32 = nullptr;
+    pn1dev::Thr* thr = nullptr; // mixed form's per-star pair thresholds
+    float* kr = nullptr;        // mixed form's per-star cancellation ratios
+    // acc and kr hold the last mixed evaluation of the uploaded run (cleared
+    // by pn1_upload and by FP64 evaluations): the mixed form's thresholds
+    // read them
+    bool acc_ok = false;
     void* part = nullptr;
     size_t part_bytes = 0;
     double* h_stage = nullptr;  // pinned, 7 * cap doubles
@@ -49,6 +56,9 @@ pn1dev::Phys make_phys(const pn1_ctx* c);
 // Launches one force evaluation (stage 0: plain, on X; 1..3: the stage
 // driver's calls, with their state updates) on the context's stream.
 int enqueue_force(pn1_ctx* c, int stage, const pn1dev::Phys& P);
+// Mixed form after an upload: one evaluation with the pair thresholds off
+// (no previous accelerations yet), so that the next has them.
+int ensure_prev_acc(pn1_ctx* c, const pn1dev::Phys& P);
 // SoA device [k][cap] -> AoS host [n][3] for k in 0..2, through the stage.
 int fetch3(pn1_ctx* c, const double* dsrc, double* out);
 // Checks that a state is resident and makes the context's device current;

@@ -5,6 +5,7 @@
 // clusters back to back; per cluster its BH first.  Tables built at upload:
 //   coff[nc + 1]   first particle of each cluster
 //   cm0[nc]        BH mass of each cluster
+//   cmmax[nc]      largest star mass of each cluster
 //   blk[nb], cblk[nc + 1]   256-star blocks of the O(n) kernels, per cluster
 //   items[nitems]  pair work items in 8 per-XCD queues (qoff[9]): whole
 //                  clusters go to the least-loaded queue, largest first;
@@ -33,6 +34,10 @@ struct pn1_ens {
     double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr, *m = nullptr, *acc = nullptr;
     double *gblk = nullptr, *gs = nullptr, *bhblk = nullptr, *eblk = nullptr, *eout = nullptr;
     double* cm0 = nullptr;
+    double* cmmax = nullptr;      // each cluster's largest star mass
+    pn1dev::Thr* thr = nullptr;   // mixed form's per-star pair thresholds
+    float* kr = nullptr;          // mixed form's per-star cancellation ratios
+    bool acc_ok = false;          // acc, kr hold the last mixed evaluation
     pn1dev::JRec64* j64 = nullptr;
     pn1dev::JRec32* j32 = nullptr;
     int *coff = nullptr, *cblk = nullptr, *qoff = nullptr;
@@ -64,6 +69,8 @@ pn1dev::Phys ens_phys(const pn1_ens* e);
 // One force evaluation of every cluster (stage 0: plain, on X; 1..3: the
 // stage driver's calls with their state updates), on the ensemble's stream.
 int ens_enqueue_force(pn1_ens* e, int stage, const pn1dev::Phys& P);
+// Mixed form after an upload: one evaluation with the pair thresholds off.
+int ens_ensure_prev_acc(pn1_ens* e, const pn1dev::Phys& P);
 int ens_fetch3(pn1_ens* e, const double* dsrc, double* out);
 bool ens_ready(pn1_ens* e, int* rc);
 void edrop_graph(pn1_ens* e);

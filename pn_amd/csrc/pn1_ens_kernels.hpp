@@ -87,11 +87,13 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
 template <int PP>
 __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr,
                                                   const JRec64* __restrict__ j64,
+                                                  const Thr* __restrict__ thr,
                                                   const double* __restrict__ st,
                                                   float* __restrict__ part,
                                                   const EItem* __restrict__ items,
                                                   unsigned int* __restrict__ counter,
-                                                  const int* __restrict__ qoff, Phys P,
+                                                  const int* __restrict__ qoff,
+                                                  const double* __restrict__ cmmax, Phys P,
                                                   int tile)
 {
     int q = 0;
@@ -100,32 +102,37 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
         if (it < 0) break;
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
-        const I32 I = load_i32(jr, j64, st, P.cap, g.i, P.eps2);
+        const I32 I = load_i32(jr, j64, thr, st, P.cap, g.i, P.eps2);
         if (!g.live) continue;                 // as in k_ens_pair64
-        float a[NACC], sN[3], cN[3];
+        float a[NACC32], sN[3], cN[3];
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
+        for (int k = 0; k < NACC32; ++k) a[k] = 0.0f;
 #pragma unroll
         for (int k = 0; k < 3; ++k) sN[k] = cN[k] = 0.0f;
         int cnt = 0;
+        const float e2 = P.eps2f;
+        const double e2d = P.eps2, ms = cmmax[w.c];
         if (g.w0 < g.w1) {
-            run32<false, PP>(jr, w.j0, g.w0, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
-            run32<true, PP>(jr, g.w0, g.w1, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
-            run32<false, PP>(jr, g.w1, w.j1, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<false, PP>(jr, j64, w.j0, g.w0, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
+            run32<true, PP>(jr, j64, g.w0, g.w1, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
+            run32<false, PP>(jr, j64, g.w1, w.j1, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
         } else {
-            run32<false, PP>(jr, w.j0, w.j1, g.i, I, a, P.eps2f, sN, cN, tile, cnt);
+            run32<false, PP>(jr, j64, w.j0, w.j1, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
         }
         fold_newton(a, sN, cN);
         if (g.li < w.nst) {
-            float* pp = part + w.pslot * (NACC + 3) + g.li;
+            float* pp = part + w.pslot * MROWS + g.li;
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) pp[(size_t)k * w.nst] = k <= A_NZ ? sN[k] : a[k];
+            for (int k = 0; k < NACC32; ++k) pp[(size_t)k * w.nst] = k <= A_NZ ? sN[k] : a[k];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pp[(size_t)(NACC + k) * w.nst] = cN[k];
+            for (int k = 0; k < 3; ++k) pp[(size_t)(NACC32 + k) * w.nst] = cN[k];
         }
     }
 }
 
+// aprev, kprev: the accelerations and cancellation ratios of the ensemble's
+// last mixed evaluation (null: none yet); cmmax: each cluster's largest
+// star mass (the mixed record's mass scale).
 __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
                                                 const double* __restrict__ m,
                                                 const double* __restrict__ cm0,
@@ -133,7 +140,10 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
                                                 const EBlk* __restrict__ blk, Phys P,
                                                 JRec64* __restrict__ j64,
                                                 JRec32* __restrict__ j32,
-                                                double* __restrict__ gblk)
+                                                double* __restrict__ gblk,
+                                                const double* aprev, const float* kprev,
+                                                const double* __restrict__ cmmax,
+                                                Thr* __restrict__ thr)
 {
     const EBlk b = blk[blockIdx.x];
     const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
@@ -141,7 +151,9 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
     double g[NGS];
 #pragma unroll
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
-    if (li < nst) prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, j32, g);
+    if (li < nst)
+        prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, j32, g, aprev, kprev,
+                  j32 ? cmmax[b.c] : 0.0, thr);
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
 }
 
@@ -181,19 +193,23 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
                                                const long long* __restrict__ poff,
                                                const double* __restrict__ gs,
                                                double* __restrict__ acc,
-                                               double* __restrict__ bhblk, Phys P, EConst E)
+                                               double* __restrict__ bhblk,
+                                               const double* __restrict__ cmmax,
+                                               float* __restrict__ kout, Phys P, EConst E)
 {
     const EBlk b = blk[blockIdx.x];
     const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
     const int li = b.first + (int)threadIdx.x;
     double cb[3] = {0.0, 0.0, 0.0};
     if (li < nst) {
-        double S_[NACC];
+        double S_[NACC], ts, a[3];
         const void* base = MODE == 0 ? (const void*)((const double*)part + poff[b.c] * NACC)
-                                     : (const void*)((const float*)part + poff[b.c] * (NACC + 3));
-        gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li, S_);
+                                     : (const void*)((const float*)part + poff[b.c] * MROWS);
+        gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li,
+                         MODE == 1 ? cmmax[b.c] : 1.0, S_, ts);
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, off + 1 + li, off, S_,
-                        gs + (size_t)b.c * NGS, fin_consts(P, cm0[b.c]), acc, cb);
+                        gs + (size_t)b.c * NGS, fin_consts(P, cm0[b.c]), acc, cb, a);
+        if (MODE == 1) kout[off + 1 + li] = cancel_ratio(ts, a);
     }
     block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
 }

@@ -63,14 +63,44 @@ __device__ __forceinline__ void stage_update(int p, int cap, const double (&a)[3
 // ---------------------------------------------------------------------
 // k_prep: per-star BH geometry + separable sums
 // ---------------------------------------------------------------------
+// Mixed form's pair thresholds of a star at x whose previous evaluation
+// gave the acceleration aprev and cancellation ratio kprev (null: none
+// known), for star masses up to mmax (see Thr).
+__device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
+                                               const double* aprev, const float* kprev,
+                                               size_t cap, int p, double mmax)
+{
+    Thr t;
+    if (!aprev) {
+        t.rc2 = 0.0f;
+        t.dn2 = __FLT_MAX__;
+        return t;
+    }
+    const double ax = aprev[p], ay = aprev[cap + p], az = aprev[2 * cap + p];
+    const double A = sqrt(ax * ax + ay * ay + az * az);
+    const double X = sqrt(x * x + y * y + z * z);
+    const double rc2 = mmax / (PN1_CLOSE_KAPPA * A);            // +inf if A == 0
+    const double dn3 = 6.0 * 0x1p-24 * X * mmax / (PN1_LAZY_ETA * A);
+    const double dn2 = cbrt(dn3 * dn3);
+    t.rc2 = (float)fmin(rc2, (double)__FLT_MAX__);
+    t.dn2 = (float)fmin(dn2, (double)__FLT_MAX__);
+    if (!(kprev[p] <= (float)PN1_HARD_K)) t.rc2 = __FLT_MAX__;   // NaN too
+    return t;
+}
+
 // Star p (SoA stride cap) of a cluster whose BH sits at index b0 and has
 // mass m0: writes its j-records and adds its terms of the separable sums
-// to g.
+// to g.  Mixed form (j32 set): also its pair thresholds, from its previous
+// acceleration aprev (SoA [3][cap]) and cancellation ratio kprev (both null
+// after an upload), and the cluster's largest star mass mmax (the mass
+// scale of the mixed record).
 __device__ __forceinline__ void prep_star(const double* __restrict__ st,
                                           const double* __restrict__ m, int cap, int p,
                                           int b0, double m0, double eps2,
                                           JRec64* __restrict__ j64, JRec32* __restrict__ j32,
-                                          double (&g)[NGS])
+                                          double (&g)[NGS], const double* aprev,
+                                          const float* kprev, double mmax,
+                                          Thr* __restrict__ thr)
 {
 #pragma clang fp contract(off)
     const double x = st[p], y = st[cap + p], z = st[2 * cap + p];
@@ -99,27 +129,34 @@ __device__ __forceinline__ void prep_star(const double* __restrict__ st,
         h.yl = (float)(y - (double)h.yh);
         h.zl = (float)(z - (double)h.zh);
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
-        h.m = (float)mp;
+        h.m = (float)(mp / mmax);
         h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
         h.Qdx = (float)r.Qdx; h.Qdy = (float)r.Qdy; h.Qdz = (float)r.Qdz;
         j32[p] = h;
+        thr[p] = pair_thresholds(x, y, z, aprev, kprev, (size_t)cap, p, mmax);
     }
     g[G_G1] = mp * e;
     g[G_M1X] = q * dx; g[G_M1Y] = q * dy; g[G_M1Z] = q * dz;
     g[G_PX] = mp * vx; g[G_PY] = mp * vy; g[G_PZ] = mp * vz;
 }
 
+// aprev, kprev: the accelerations and cancellation ratios of the context's
+// last mixed evaluation, read before this evaluation's k_fin overwrites
+// them (null: none yet).
 __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
                                             const double* __restrict__ m, Phys P,
                                             JRec64* __restrict__ j64,
                                             JRec32* __restrict__ j32,
-                                            double* __restrict__ gblk)
+                                            double* __restrict__ gblk,
+                                            const double* aprev, const float* kprev,
+                                            Thr* __restrict__ thr)
 {
     const int p = blockIdx.x * WG + threadIdx.x + 1;
     double g[NGS];
 #pragma unroll
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
-    if (p < P.n) prep_star(st, m, P.cap, p, 0, P.m0, P.eps2, j64, j32, g);
+    if (p < P.n)
+        prep_star(st, m, P.cap, p, 0, P.m0, P.eps2, j64, j32, g, aprev, kprev, P.mmax, thr);
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
 }
 
@@ -267,21 +304,26 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
 
 // ---------------------------------------------------------------------
 // force pair kernel, mixed: FP32 pair arithmetic on double-single
-// positions, FP32 tile sums folded into Kahan-compensated FP32 totals
+// positions, FP32 tile sums folded into Kahan-compensated FP32 totals, and
+// the Newtonian term of the few pairs that can dominate a star's
+// acceleration in FP64
 // ---------------------------------------------------------------------
 struct I32 {
-    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2e;
+    float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2e, rc2, dn2;
 };
 
-// Star i: its mixed j-record, velocity and |d_i|^2 (from its FP64 record).
+// Star i: its mixed j-record, velocity, |d_i|^2 (from its FP64 record) and
+// pair thresholds.
 __device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
                                         const JRec64* __restrict__ j64,
+                                        const Thr* __restrict__ thr,
                                         const double* __restrict__ st, int cap, int i,
                                         double eps2)
 {
     const JRec32& r = jr[i];
     const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
     const double D2 = j64[i].D2;
+    const Thr t = thr[i];
     I32 I;
     I.xh = r.xh; I.yh = r.yh; I.zh = r.zh;
     I.xl = r.xl; I.yl = r.yl; I.zl = r.zl;
@@ -289,12 +331,93 @@ __device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
     I.hx = (float)(PN1_4OSQ15 * vx); I.hy = (float)(PN1_4OSQ15 * vy);
     I.hz = (float)(PN1_4OSQ15 * vz);
     I.D2e = (float)(D2 + eps2);
+    I.rc2 = t.rc2;
+    I.dn2 = t.dn2;
     return I;
 }
 
+// Folds the short-run FP32 Newtonian sums into their Kahan totals (total =
+// sN - cN).
+__device__ __forceinline__ void fold_newton(float (&a)[NACC32], float (&sN)[3], float (&cN)[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma clang fp contract(off)
+        const float y = a[A_NX + k] - cN[k];
+        const float u = sN[k] + y;
+        cN[k] = (u - sN[k]) - y;
+        sN[k] = u;
+        a[A_NX + k] = 0.0f;
+    }
+}
+
+// u = fl(s + y), and its rounding error (Knuth's TwoSum, exact for any
+// magnitudes) taken off the compensation c of a total kept as s - c.
+__device__ __forceinline__ void two_sum_into(float s, float y, float& u, float& c)
+{
+#pragma clang fp contract(off)
+    u = s + y;
+    const float bv = u - s;
+    const float av = u - bv;
+    c -= (s - av) + (y - bv);
+}
+
+// Close pairs (R2e < rc2, not the self pair), taken when any lane of the
+// wave has one; `close` selects the lanes.  A close pair's Newtonian term
+// (m_j / mscale) w^3 r is evaluated in FP64, from j's FP64 record and the lane's
+// double-single position, and goes into the Kahan total; the FP32 term
+// mw3 r that the caller adds next is cancelled exactly: the short-run sum
+// is folded, then set to -p with p = fl(mw3 r), so the caller's
+// fma(mw3, r, -p) leaves the exact product residual e = mw3 r - p, and
+// T64 - e enters the total (as hi + lo floats: hi by TwoSum, lo into the
+// compensation).  Every lane computes; the other lanes keep their sums.
+// The branch around it is wave-uniform, so j and the record stream stay
+// scalar.
+__device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const I32& I,
+                                            double eps2, double mscale, float mw3, float rx,
+                                            float ry, float rz, float (&a)[NACC32],
+                                            float (&sN)[3], float (&cN)[3])
+{
+#pragma clang fp contract(off)
+    // the lane's position, made opaque here so that its FP64 form is built
+    // in this rarely taken branch rather than kept live across the j loop
+    float ph[3] = {I.xh, I.yh, I.zh}, pl[3] = {I.xl, I.yl, I.zl};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) __asm__ volatile("" : "+v"(ph[k]), "+v"(pl[k]));
+    const double dx = J.x - ((double)ph[0] + (double)pl[0]);
+    const double dy = J.y - ((double)ph[1] + (double)pl[1]);
+    const double dz = J.z - ((double)ph[2] + (double)pl[2]);
+    const double R2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+    const double w = rsq64(R2);
+    const double mw3d = (J.m / mscale) * (w * (w * w));   // the record's mass units
+    const double d[3] = {dx, dy, dz};
+    const float r[3] = {rx, ry, rz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float p = mw3 * r[k];
+        const float e = __builtin_fmaf(mw3, r[k], -p);
+        const double D = mw3d * d[k] - (double)e;
+        const float hi = (float)D;
+        const float lo = (float)(D - (double)hi);
+        // total = s - c; the short-run sum, then hi + lo, are added with
+        // TwoSum, exact whatever the magnitudes (Kahan's update is exact
+        // only while the running sum is the larger addend, and a star whose
+        // terms cancel to a small |a| breaks that)
+        float s1, c1 = cN[k];
+        two_sum_into(sN[k], a[A_NX + k], s1, c1);
+        float s2 = s1;
+        two_sum_into(s1, hi, s2, c1);
+        c1 -= lo;
+        a[A_NX + k] = close ? -p : a[A_NX + k];
+        sN[k] = close ? s2 : sN[k];
+        cN[k] = close ? c1 : cN[k];
+    }
+}
+
 template <bool MASK, bool RF>
-__device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32& I,
-                                       float (&a)[NACC], float eps2)
+__device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict__ j64, int j,
+                                       int i, const I32& I, float (&a)[NACC32], float (&sN)[3],
+                                       float (&cN)[3], float eps2, double eps2d, double mscale)
 {
     float mj = J.m, MBj = J.MB, Qdx = J.Qdx, Qdy = J.Qdy, Qdz = J.Qdz;
     if (MASK) {
@@ -305,11 +428,20 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
         Qdy = self ? 0.0f : Qdy;
         Qdz = self ? 0.0f : Qdz;
     }
-    // separation from the hi and lo parts: close pairs keep their digits
-    const float rx = (J.xh - I.xh) + (J.xl - I.xl);
-    const float ry = (J.yh - I.yh) + (J.yl - I.yl);
-    const float rz = (J.zh - I.zh) + (J.zl - I.zl);
-    const float R2e = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+    // Separation from the hi parts of the positions; inside the lane's dn2
+    // the lo parts are added, which gives the double-single separation
+    // (x_hi,j - x_hi,i) + (x_lo,j - x_lo,i), so close pairs keep their digits.
+    float rx = J.xh - I.xh, ry = J.yh - I.yh, rz = J.zh - I.zh;
+    float R2e = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+    if (R2e < I.dn2) {
+        rx += J.xl - I.xl;
+        ry += J.yl - I.yl;
+        rz += J.zl - I.zl;
+        R2e = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+    }
+    // (the FP64 term of a close pair takes its own separation, so a close
+    // pair outside dn2 is still exact)
+    const bool close = R2e < I.rc2 && (!MASK || j != i);
     float w = __builtin_amdgcn_rsqf(R2e);
     if (RF) {
         // one Newton step: v_rsq_f32 (~1.6 ulp) -> ~1 ulp
@@ -319,11 +451,9 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     const float w2 = w * w;
     const float mw = mj * w;
     const float mw3 = mw * w2;
-    a[A_NX] = __builtin_fmaf(mw3, rx, a[A_NX]);
-    a[A_NY] = __builtin_fmaf(mw3, ry, a[A_NY]);
-    a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
     a[A_PHI] += mw;
     a[A_W3] += mw3;
+    a[A_TS] = __builtin_fmaf(mw, w, a[A_TS]);
     a[A_TX] = __builtin_fmaf(w, Qdx, a[A_TX]);
     a[A_TY] = __builtin_fmaf(w, Qdy, a[A_TY]);
     a[A_TZ] = __builtin_fmaf(w, Qdz, a[A_TZ]);
@@ -346,34 +476,27 @@ __device__ __forceinline__ void body32(const JRec32& J, int j, int i, const I32&
     a[A_UY] = __builtin_fmaf(Ku, J.sy, a[A_UY]);
     a[A_UZ] = __builtin_fmaf(Ku, J.sz, a[A_UZ]);
     a[A_KS] += Ku;
-}
-
-// Folds the short-run FP32 Newtonian sums into their Kahan totals.
-__device__ __forceinline__ void fold_newton(float (&a)[NACC], float (&sN)[3], float (&cN)[3])
-{
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-#pragma clang fp contract(off)
-        const float y = a[A_NX + k] - cN[k];
-        const float u = sN[k] + y;
-        cN[k] = (u - sN[k]) - y;
-        sN[k] = u;
-        a[A_NX + k] = 0.0f;
-    }
+    if (__builtin_amdgcn_ballot_w64(close))
+        close_pairs(close, j64[j], I, eps2d, mscale, mw3, rx, ry, rz, a, sN, cN);
+    a[A_NX] = __builtin_fmaf(mw3, rx, a[A_NX]);
+    a[A_NY] = __builtin_fmaf(mw3, ry, a[A_NY]);
+    a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
 }
 
 // Mixed-form j loop: as run64, plus a wave-uniform pair counter that folds
 // the Newtonian sums into Kahan totals every `tile` pairs without
 // restarting the record stream.
 template <bool MASK, int PP>
-__device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int j1, int i,
-                                      const I32& I, float (&a)[NACC], float eps2,
-                                      float (&sN)[3], float (&cN)[3], int tile, int& cnt)
+__device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
+                                      const JRec64* __restrict__ j64, int j0, int j1, int i,
+                                      const I32& I, float (&a)[NACC32], float eps2, double eps2d,
+                                      double mscale, float (&sN)[3], float (&cN)[3], int tile,
+                                      int& cnt)
 {
     constexpr bool RF = (PP & 2) != 0;
     if ((PP & 1) == 0) {
         for (int j = j0; j < j1; ++j) {
-            body32<MASK, RF>(jr[j], j, i, I, a, eps2);
+            body32<MASK, RF>(jr[j], j64, j, i, I, a, sN, cN, eps2, eps2d, mscale);
             if (++cnt >= tile) {
                 cnt = 0;
                 fold_newton(a, sN, cN);
@@ -386,9 +509,9 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int
     int j = j0;
     for (; j + 1 < j1; j += 2) {
         const JRec32 rb = jr[j + 1];
-        body32<MASK, RF>(ra, j, i, I, a, eps2);
+        body32<MASK, RF>(ra, j64, j, i, I, a, sN, cN, eps2, eps2d, mscale);
         ra = jr[j + 2];
-        body32<MASK, RF>(rb, j + 1, i, I, a, eps2);
+        body32<MASK, RF>(rb, j64, j + 1, i, I, a, sN, cN, eps2, eps2d, mscale);
         cnt += 2;
         if (cnt >= tile) {
             cnt = 0;
@@ -396,7 +519,7 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int
         }
     }
     if (j < j1) {
-        body32<MASK, RF>(ra, j, i, I, a, eps2);
+        body32<MASK, RF>(ra, j64, j, i, I, a, sN, cN, eps2, eps2d, mscale);
         ++cnt;
     }
 }
@@ -404,13 +527,15 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr, int j0, int
 
 // The Newtonian sums are Kahan-compensated every `tile` pairs (at least
 // 2): they carry the stars' accelerations, and plain FP32 sums over longer
-// runs are the largest error of this form (measured at n = 131073: max
-// per-star error 8e-7 compensating every 32 pairs, 1.5e-6 every 256).  The other sums
-// enter scaled by 1/c^2 (or, for the BH, beside its FP64 Newtonian term)
-// and accumulate in plain FP32 over the whole chunk.
+// runs are the largest error of this form after the close pairs (measured
+// at n = 131073: max per-star error 8e-7 compensating every 32 pairs,
+// 1.5e-6 every 256).  The other sums enter scaled by 1/c^2 (or, for the
+// BH, beside its FP64 Newtonian term) and accumulate in plain FP32 over
+// the whole chunk.
 template <int PP>
 __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
                                               const JRec64* __restrict__ j64,
+                                              const Thr* __restrict__ thr,
                                               const double* __restrict__ st,
                                               float* __restrict__ part, Phys P, int S,
                                               int nIblk, int ib0, int tile)
@@ -422,31 +547,32 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(jr, j64, st, P.cap, i, P.eps2);
+    const I32 I = load_i32(jr, j64, thr, st, P.cap, i, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
-    float a[NACC], sumN[3], cmpN[3];
+    const double eps2d = P.eps2, mscale = P.mmax;
+    float a[NACC32], sumN[3], cmpN[3];
 #pragma unroll
-    for (int k = 0; k < NACC; ++k) a[k] = 0.0f;
+    for (int k = 0; k < NACC32; ++k) a[k] = 0.0f;
 #pragma unroll
     for (int k = 0; k < 3; ++k) sumN[k] = cmpN[k] = 0.0f;
     int cnt = 0;
     if (R.w0 < R.w1) {
-        run32<false, PP>(jr, R.j0, R.w0, i, I, a, eps2, sumN, cmpN, tile, cnt);
-        run32<true, PP>(jr, R.w0, R.w1, i, I, a, eps2, sumN, cmpN, tile, cnt);
-        run32<false, PP>(jr, R.w1, R.j1, i, I, a, eps2, sumN, cmpN, tile, cnt);
+        run32<false, PP>(jr, j64, R.j0, R.w0, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
+        run32<true, PP>(jr, j64, R.w0, R.w1, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
+        run32<false, PP>(jr, j64, R.w1, R.j1, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
     } else {
-        run32<false, PP>(jr, R.j0, R.j1, i, I, a, eps2, sumN, cmpN, tile, cnt);
+        run32<false, PP>(jr, j64, R.j0, R.j1, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
     }
     fold_newton(a, sumN, cmpN);
     if (li < n) {
-        // [s][NACC + 3][cap]: the sums, then the Newtonian compensations
+        // [s][MROWS][cap]: the sums, then the Newtonian compensations
         const size_t cap = (size_t)P.cap;
-        float* ps = part + (size_t)s * (NACC + 3) * cap + li;
+        float* ps = part + (size_t)s * MROWS * cap + li;
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) ps[k * cap] = k <= A_NZ ? sumN[k] : a[k];
+        for (int k = 0; k < NACC32; ++k) ps[k * cap] = k <= A_NZ ? sumN[k] : a[k];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) ps[(NACC + k) * cap] = cmpN[k];
+        for (int k = 0; k < 3; ++k) ps[(NACC32 + k) * cap] = cmpN[k];
     }
 }
 
@@ -455,14 +581,17 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
 // ---------------------------------------------------------------------
 // Sum of the S chunk partials of one star.  `part` points at the
 // cluster's partial area, `stride` is its per-sum stride and `idx` the
-// star's index in it.  FP64: [S][NACC][stride]; mixed: [S][NACC + 3][stride]
-// floats, the last 3 rows holding the Newtonian Kahan compensations.
+// star's index in it.  FP64: [S][NACC][stride]; mixed: [S][MROWS][stride]
+// floats (the NACC32 sums, then the Newtonian Kahan compensations), whose
+// sums built from the record's mass m_j / mscale are scaled back by mscale;
+// ts is the mixed form's A_TS sum (0 for FP64).
 template <int MODE>
 __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S, size_t stride,
-                                           int idx, double (&A)[NACC])
+                                           int idx, double mscale, double (&A)[NACC], double& ts)
 {
 #pragma unroll
     for (int k = 0; k < NACC; ++k) A[k] = 0.0;
+    ts = 0.0;
     if (MODE == 0) {
         const double* pp = (const double*)part;
         for (int s = 0; s < S; ++s)
@@ -471,13 +600,25 @@ __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S,
     } else {
         const float* pp = (const float*)part;
         for (int s = 0; s < S; ++s) {
-            const float* ps = pp + (size_t)s * (NACC + 3) * stride + idx;
+            const float* ps = pp + (size_t)s * MROWS * stride + idx;
 #pragma unroll
             for (int k = 0; k < NACC; ++k) A[k] += (double)ps[(size_t)k * stride];
+            ts += (double)ps[(size_t)A_TS * stride];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) A[A_NX + k] -= (double)ps[(size_t)(NACC + k) * stride];
+            for (int k = 0; k < 3; ++k) A[A_NX + k] -= (double)ps[(size_t)(NACC32 + k) * stride];
         }
+#pragma unroll
+        for (int k = 0; k < NACC; ++k)
+            if (k <= A_UZ || k == A_KS || k == A_PHI || k == A_W3) A[k] *= mscale;
+        ts *= mscale;
     }
+}
+
+// Mixed form: the star's cancellation ratio for the next evaluation's
+// thresholds, sum m_j w^2 / |a| (+inf for a = 0).
+__device__ __forceinline__ float cancel_ratio(double ts, const double (&a)[3])
+{
+    return (float)(ts / sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]));
 }
 
 // Run constants shared by the O(n) passes of one cluster.
@@ -493,7 +634,8 @@ __device__ __forceinline__ void fin_star(const double* in, double* X, double* Y,
                                          double* H2, const double* __restrict__ m, int cap,
                                          int p, int b0, const double (&S_)[NACC],
                                          const double* __restrict__ gs, const FinC& C,
-                                         double* __restrict__ acc, double (&cb)[3])
+                                         double* __restrict__ acc, double (&cb)[3],
+                                         double (&a)[3])
 {
     const double m0 = C.m0, mp = m[p], eps2 = C.eps2;
     const double x[3] = {in[p], in[cap + p], in[2 * cap + p]};
@@ -520,7 +662,6 @@ __device__ __forceinline__ void fin_star(const double* in, double* X, double* Y,
     const double c1 = -m0 * E3 * (4.0 * S_[A_PHI] + 1.25 * (G1 - mp * e)) + E3 * S_[A_ZE];
     const double cB = 4.0 * m0 * e - V2;   // also the i-only part of the K_r bracket
     const double cS = 5.0 * m0 * mp * e * e * e * e - 4.0 * mp * E3 * V2;
-    double a[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const double newton = S_[A_NX + k] - m0 * E3 * d[k];
@@ -581,20 +722,24 @@ __device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
 
 // Stars p0 .. p1 - 1 (p0 = 1, p1 = n: every star; a sub-range is one i shard
 // of a plain evaluation, STAGE 0).
+// kout (mixed form): each star's cancellation ratio, for the next
+// evaluation's pair thresholds.
 template <int MODE, int STAGE>
 __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double* Y, double* H1,
                                            double* H2, const double* __restrict__ m,
                                            const void* __restrict__ part, int S,
                                            const double* __restrict__ gs,
                                            double* __restrict__ acc,
-                                           double* __restrict__ bhblk, Phys P, int p0, int p1)
+                                           double* __restrict__ bhblk, float* __restrict__ kout,
+                                           Phys P, int p0, int p1)
 {
     const int p = p0 + blockIdx.x * WG + threadIdx.x;
     double cb[3] = {0.0, 0.0, 0.0};
     if (p < p1) {
-        double S_[NACC];
-        gather_acc<MODE>(part, S, (size_t)P.cap, p, S_);
-        fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb);
+        double S_[NACC], ts, a[3];
+        gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
+        fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb, a);
+        if (MODE == 1) kout[p] = cancel_ratio(ts, a);
     }
     block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
 }
