@@ -100,7 +100,7 @@ void plan(pn1_ctx* c)
     } else {
         c->pair_grid = S * c->nIblk;
     }
-    c->tile = c->tile_req > 0 ? (c->tile_req < 2 ? 2 : c->tile_req) : 16;
+    c->tile = c->tile_req > 0 ? (c->tile_req < 2 ? 2 : c->tile_req) : 32;
 }
 
 // Partial-sum buffer for a j split of S chunks (default: the plan's).
@@ -126,11 +126,12 @@ int ensure_part(pn1_ctx* c, int split = 0)
 }
 
 // Kernel form in use: the requested one, or per mode the faster one
-// measured at n = 131073 (FP64: one record at a time; mixed: two in flight).
+// measured at n = 131073: one record at a time in both modes (the mixed
+// kernel then fits 8 waves per SIMD).
 int kernel_variant(const pn1_ctx* c)
 {
     if (c->variant_req >= 0) return c->mode == PN1_MIXED ? c->variant_req : (c->variant_req & 1);
-    return c->mode == PN1_MIXED ? 1 : 0;
+    return 0;
 }
 
 void drop_graph(pn1_ctx* c)
@@ -161,16 +162,37 @@ bool ready(pn1_ctx* c, int* rc)
 template <int MODE, int STAGE>
 void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
 {
+    // the cancellation ratios are refreshed by plain evaluations and the
+    // first stage of each step (the pair kernel accumulated their sums)
+    float* kout = (MODE == 1 && STAGE <= 1) ? c->kr : nullptr;
     hipLaunchKernelGGL((k_fin<MODE, STAGE>), dim3(c->nblk), dim3(WG), 0, c->st, in, c->X, c->Y,
                        c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk,
-                       c->kr, P, 1, c->n);
+                       kout, P, 1, c->n);
     hipLaunchKernelGGL((k_bh<STAGE>), dim3(1), dim3(WG), 0, c->st, c->bhblk, c->nblk, c->gs, c->X,
                        c->Y, c->H1, c->H2, c->acc, P);
 }
 
+template <int PP, bool TS, bool EQ>
+void launch32(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S, dim3 g)
+{
+    hipLaunchKernelGGL((k_pair32<PP, TS, EQ>), g, dim3(WG), 0, c->st, c->j32, c->j64, c->thr, in,
+                       (float*)c->part, P, S, nib, ib0, c->tile);
+}
+
+// mixed pair kernels by [variant][cancellation sums][equal masses]
+using Launch32 = void (*)(pn1_ctx*, const double*, const Phys&, int, int, int, dim3);
+const Launch32 k_launch32[16] = {
+    launch32<0, false, false>, launch32<0, false, true>, launch32<0, true, false>,
+    launch32<0, true, true>,   launch32<1, false, false>, launch32<1, false, true>,
+    launch32<1, true, false>,  launch32<1, true, true>,   launch32<2, false, false>,
+    launch32<2, false, true>,  launch32<2, true, false>,  launch32<2, true, true>,
+    launch32<3, false, false>, launch32<3, false, true>,  launch32<3, true, false>,
+    launch32<3, true, true>};
+
 // The pair kernel over i blocks ib0 .. ib0 + nib - 1 against every star,
-// with the j range split in S chunks.
-void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S)
+// with the j range split in S chunks.  ts (mixed form): also accumulate the
+// sums behind the stars' cancellation ratios.
+void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S, bool ts)
 {
     const bool mixed = c->mode == PN1_MIXED;
     int grid = S * nib;
@@ -181,17 +203,7 @@ void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, 
     const dim3 g(grid), blk(WG);
     const int variant = kernel_variant(c);
     if (mixed) {
-        float* pp = (float*)c->part;
-        switch (variant) {
-        case 0: hipLaunchKernelGGL(k_pair32<0>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
-                                   c->tile); break;
-        case 1: hipLaunchKernelGGL(k_pair32<1>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
-                                   c->tile); break;
-        case 2: hipLaunchKernelGGL(k_pair32<2>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
-                                   c->tile); break;
-        default: hipLaunchKernelGGL(k_pair32<3>, g, blk, 0, c->st, c->j32, c->j64, c->thr, in, pp, P, S, nib, ib0,
-                                    c->tile); break;
-        }
+        k_launch32[(variant & 3) * 4 + (ts ? 2 : 0) + (c->eqm ? 1 : 0)](c, in, P, ib0, nib, S, g);
     } else if (variant & 1) {
         hipLaunchKernelGGL(k_pair64<1>, g, blk, 0, c->st, c->j64, in, (double*)c->part, P, S, nib,
                            ib0);
@@ -214,7 +226,7 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
     if (c->n > 1) {
         hipEvent_t e1 = nullptr;
         if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
-        launch_pair(c, in, P, 0, c->nIblk, c->S);
+        launch_pair(c, in, P, 0, c->nIblk, c->S, stage <= 1);
         if (e1) HIPCHK(c, hipEventRecord(e1, c->st));
     }
     switch (stage * 2 + (mixed ? 1 : 0)) {
@@ -420,7 +432,11 @@ int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, in
     c->n = n;
     c->m0 = m[0];
     c->mmax = 0.0;
-    for (int i = 1; i < n; ++i) c->mmax = std::max(c->mmax, m[i]);
+    c->eqm = true;
+    for (int i = 1; i < n; ++i) {
+        c->mmax = std::max(c->mmax, m[i]);
+        c->eqm = c->eqm && m[i] == m[1];
+    }
     c->acc_ok = false;
     c->have_state = true;
     plan(c);
@@ -483,7 +499,7 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     hipEvent_t e1 = nullptr;
     if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
-    launch_pair(c, in, P, (i0 - 1) / WG, nib, S);
+    launch_pair(c, in, P, (i0 - 1) / WG, nib, S, true);
     if (e1) HIPCHK(c, hipEventRecord(e1, c->st));
     const dim3 gf(nib);
     if (mixed)

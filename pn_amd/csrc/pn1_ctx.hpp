@@ -21,9 +21,10 @@ struct pn1_ctx {
     int mode = PN1_FP64;
     double m0 = 0.0;            // host copy of m[0]
     double mmax = 0.0;          // largest star mass (mixed form's pair thresholds)
+    bool eqm = false;           // every star has the same mass
     int jsplit_req = 0, tile_req = 0, variant_req = -1;
     // launch plan for the current n
-    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 16;
+    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 32;
     // device buffers
     double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr;
     double *m = nullptr, *acc = nullptr, *gblk = nullptr, *gs = nullptr;

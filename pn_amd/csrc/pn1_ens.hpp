@@ -30,7 +30,7 @@ struct pn1_ens {
     double eps = 1e-3, c = 1.0, dt = 1.0 / 1024.0;
     int mode = PN1_FP64;
     int L_req = 0, tile_req = 0, variant_req = -1;
-    int L = 512, tile = 16;
+    int L = 512, tile = 32;
     double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr, *m = nullptr, *acc = nullptr;
     double *gblk = nullptr, *gs = nullptr, *bhblk = nullptr, *eblk = nullptr, *eout = nullptr;
     double* cm0 = nullptr;
@@ -38,6 +38,7 @@ struct pn1_ens {
     pn1dev::Thr* thr = nullptr;   // mixed form's per-star pair thresholds
     float* kr = nullptr;          // mixed form's per-star cancellation ratios
     bool acc_ok = false;          // acc, kr hold the last mixed evaluation
+    bool eqm = false;             // each cluster's stars have one mass
     pn1dev::JRec64* j64 = nullptr;
     pn1dev::JRec32* j32 = nullptr;
     int *coff = nullptr, *cblk = nullptr, *qoff = nullptr;

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
     }
 }
 
-template <int PP>
+template <int PP, bool TS, bool EQ>
 __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr,
                                                   const JRec64* __restrict__ j64,
                                                   const Thr* __restrict__ thr,
@@ -111,19 +111,20 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
         for (int k = 0; k < 3; ++k) sN[k] = cN[k] = 0.0f;
         int cnt = 0;
         const float e2 = P.eps2f;
-        const double e2d = P.eps2, ms = cmmax[w.c];
+        const double e2d = P.eps2, rs = 1.0 / cmmax[w.c];
         if (g.w0 < g.w1) {
-            run32<false, PP>(jr, j64, w.j0, g.w0, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
-            run32<true, PP>(jr, j64, g.w0, g.w1, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
-            run32<false, PP>(jr, j64, g.w1, w.j1, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
+            run32<false, PP, TS, EQ>(jr, j64, w.j0, g.w0, g.i, I, a, e2, e2d, rs, sN, cN, tile, cnt);
+            run32<true, PP, TS, EQ>(jr, j64, g.w0, g.w1, g.i, I, a, e2, e2d, rs, sN, cN, tile, cnt);
+            run32<false, PP, TS, EQ>(jr, j64, g.w1, w.j1, g.i, I, a, e2, e2d, rs, sN, cN, tile, cnt);
         } else {
-            run32<false, PP>(jr, j64, w.j0, w.j1, g.i, I, a, e2, e2d, ms, sN, cN, tile, cnt);
+            run32<false, PP, TS, EQ>(jr, j64, w.j0, w.j1, g.i, I, a, e2, e2d, rs, sN, cN, tile, cnt);
         }
         fold_newton(a, sN, cN);
         if (g.li < w.nst) {
             float* pp = part + w.pslot * MROWS + g.li;
 #pragma unroll
-            for (int k = 0; k < NACC32; ++k) pp[(size_t)k * w.nst] = k <= A_NZ ? sN[k] : a[k];
+            for (int k = 0; k < NACC32; ++k)
+                if (TS || k != A_TS) pp[(size_t)k * w.nst] = k <= A_NZ ? sN[k] : a[k];
 #pragma unroll
             for (int k = 0; k < 3; ++k) pp[(size_t)(NACC32 + k) * w.nst] = cN[k];
         }
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
                          MODE == 1 ? cmmax[b.c] : 1.0, S_, ts);
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, off + 1 + li, off, S_,
                         gs + (size_t)b.c * NGS, fin_consts(P, cm0[b.c]), acc, cb, a);
-        if (MODE == 1) kout[off + 1 + li] = cancel_ratio(ts, a);
+        if (MODE == 1 && kout) kout[off + 1 + li] = cancel_ratio(ts, a);
     }
     block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
 }

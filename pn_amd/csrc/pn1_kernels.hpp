@@ -83,8 +83,9 @@ __device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
     const double dn3 = 6.0 * 0x1p-24 * X * mmax / (PN1_LAZY_ETA * A);
     const double dn2 = cbrt(dn3 * dn3);
     t.rc2 = (float)fmin(rc2, (double)__FLT_MAX__);
-    t.dn2 = (float)fmin(dn2, (double)__FLT_MAX__);
     if (!(kprev[p] <= (float)PN1_HARD_K)) t.rc2 = __FLT_MAX__;   // NaN too
+    // close pairs are found inside the lo-part branch: dn2 >= rc2
+    t.dn2 = fmaxf((float)fmin(dn2, (double)__FLT_MAX__), t.rc2);
     return t;
 }
 
@@ -364,8 +365,9 @@ __device__ __forceinline__ void two_sum_into(float s, float y, float& u, float& 
 
 // Close pairs (R2e < rc2, not the self pair), taken when any lane of the
 // wave has one; `close` selects the lanes.  A close pair's Newtonian term
-// (m_j / mscale) w^3 r is evaluated in FP64, from j's FP64 record and the lane's
-// double-single position, and goes into the Kahan total; the FP32 term
+// (m_j / mscale) w^3 r is evaluated in FP64 (rscale = 1 / mscale), from
+// j's FP64 record and the lane's double-single position, and goes into the
+// Kahan total; the FP32 term
 // mw3 r that the caller adds next is cancelled exactly: the short-run sum
 // is folded, then set to -p with p = fl(mw3 r), so the caller's
 // fma(mw3, r, -p) leaves the exact product residual e = mw3 r - p, and
@@ -374,13 +376,14 @@ __device__ __forceinline__ void two_sum_into(float s, float y, float& u, float& 
 // The branch around it is wave-uniform, so j and the record stream stay
 // scalar.
 __device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const I32& I,
-                                            double eps2, double mscale, float mw3, float rx,
+                                            double eps2, double rscale, float mw3, float rx,
                                             float ry, float rz, float (&a)[NACC32],
                                             float (&sN)[3], float (&cN)[3])
 {
 #pragma clang fp contract(off)
-    // the lane's position, made opaque here so that its FP64 form is built
-    // in this rarely taken branch rather than kept live across the j loop
+    // the lane's position (double-single: hi + lo is x to 2^-48), made
+    // opaque here so that its FP64 form is built in this rarely taken branch
+    // rather than hoisted out of the j loop and kept live across it
     float ph[3] = {I.xh, I.yh, I.zh}, pl[3] = {I.xl, I.yl, I.zl};
 #pragma unroll
     for (int k = 0; k < 3; ++k) __asm__ volatile("" : "+v"(ph[k]), "+v"(pl[k]));
@@ -389,7 +392,7 @@ __device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const I
     const double dz = J.z - ((double)ph[2] + (double)pl[2]);
     const double R2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
     const double w = rsq64(R2);
-    const double mw3d = (J.m / mscale) * (w * (w * w));   // the record's mass units
+    const double mw3d = (J.m * rscale) * (w * (w * w));   // the record's mass units
     const double d[3] = {dx, dy, dz};
     const float r[3] = {rx, ry, rz};
 #pragma unroll
@@ -414,10 +417,27 @@ __device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const I
     }
 }
 
-template <bool MASK, bool RF>
+// FP32 reciprocal square root of the pair kernels (RF: one Newton step,
+// v_rsq_f32 ~1.6 ulp -> ~1 ulp).
+template <bool RF>
+__device__ __forceinline__ float rsq32(float R2e)
+{
+    float w = __builtin_amdgcn_rsqf(R2e);
+    if (RF) {
+        const float r = __builtin_fmaf(-(R2e * w), w, 1.0f);
+        w = __builtin_fmaf(0.5f * w, r, w);
+    }
+    return w;
+}
+
+// One pair.  TS: also accumulate A_TS (the evaluations that refresh the
+// stars' cancellation ratios).  EQ: every star of the cluster(s) has the
+// same mass, so the record's mass m_j / mscale is 1 exactly (the self pair
+// of the masked band keeps its zero mass) and m_j w is w.
+template <bool MASK, bool RF, bool TS, bool EQ>
 __device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict__ j64, int j,
                                        int i, const I32& I, float (&a)[NACC32], float (&sN)[3],
-                                       float (&cN)[3], float eps2, double eps2d, double mscale)
+                                       float (&cN)[3], float eps2, double eps2d, double rscale)
 {
     float mj = J.m, MBj = J.MB, Qdx = J.Qdx, Qdy = J.Qdy, Qdz = J.Qdz;
     if (MASK) {
@@ -438,22 +458,23 @@ __device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict
         ry += J.yl - I.yl;
         rz += J.zl - I.zl;
         R2e = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, __builtin_fmaf(rz, rz, eps2)));
+        // close pairs (dn2 >= rc2, so they are all in this branch): their
+        // FP32 mw3 is formed here as below, bit for bit
+        const bool close = R2e < I.rc2 && (!MASK || j != i);
+        if (__builtin_amdgcn_ballot_w64(close)) {
+            const float wc = rsq32<RF>(R2e);
+            const float mw3c = (EQ && !MASK ? wc : mj * wc) * (wc * wc);
+            close_pairs(close, j64[j], I, eps2d, rscale, mw3c, rx, ry, rz, a, sN, cN);
+        }
     }
-    // (the FP64 term of a close pair takes its own separation, so a close
-    // pair outside dn2 is still exact)
-    const bool close = R2e < I.rc2 && (!MASK || j != i);
-    float w = __builtin_amdgcn_rsqf(R2e);
-    if (RF) {
-        // one Newton step: v_rsq_f32 (~1.6 ulp) -> ~1 ulp
-        const float r = __builtin_fmaf(-(R2e * w), w, 1.0f);
-        w = __builtin_fmaf(0.5f * w, r, w);
-    }
+    const float w = rsq32<RF>(R2e);
     const float w2 = w * w;
-    const float mw = mj * w;
+    constexpr bool UNIT = EQ && !MASK;
+    const float mw = UNIT ? w : mj * w;
     const float mw3 = mw * w2;
     a[A_PHI] += mw;
     a[A_W3] += mw3;
-    a[A_TS] = __builtin_fmaf(mw, w, a[A_TS]);
+    if (TS) a[A_TS] = UNIT ? a[A_TS] + w2 : __builtin_fmaf(mw, w, a[A_TS]);
     a[A_TX] = __builtin_fmaf(w, Qdx, a[A_TX]);
     a[A_TY] = __builtin_fmaf(w, Qdy, a[A_TY]);
     a[A_TZ] = __builtin_fmaf(w, Qdz, a[A_TZ]);
@@ -476,27 +497,26 @@ __device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict
     a[A_UY] = __builtin_fmaf(Ku, J.sy, a[A_UY]);
     a[A_UZ] = __builtin_fmaf(Ku, J.sz, a[A_UZ]);
     a[A_KS] += Ku;
-    if (__builtin_amdgcn_ballot_w64(close))
-        close_pairs(close, j64[j], I, eps2d, mscale, mw3, rx, ry, rz, a, sN, cN);
     a[A_NX] = __builtin_fmaf(mw3, rx, a[A_NX]);
     a[A_NY] = __builtin_fmaf(mw3, ry, a[A_NY]);
     a[A_NZ] = __builtin_fmaf(mw3, rz, a[A_NZ]);
 }
 
-// Mixed-form j loop: as run64, plus a wave-uniform pair counter that folds
-// the Newtonian sums into Kahan totals every `tile` pairs without
-// restarting the record stream.
-template <bool MASK, int PP>
+// Mixed-form j loop: PP bit 0 clear: one body at a time; set: two bodies
+// per iteration, records two ahead (as run64).
+// A wave-uniform pair counter folds the Newtonian sums into Kahan totals
+// every `tile` pairs without restarting the record stream.
+template <bool MASK, int PP, bool TS, bool EQ>
 __device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
                                       const JRec64* __restrict__ j64, int j0, int j1, int i,
                                       const I32& I, float (&a)[NACC32], float eps2, double eps2d,
-                                      double mscale, float (&sN)[3], float (&cN)[3], int tile,
+                                      double rscale, float (&sN)[3], float (&cN)[3], int tile,
                                       int& cnt)
 {
     constexpr bool RF = (PP & 2) != 0;
     if ((PP & 1) == 0) {
         for (int j = j0; j < j1; ++j) {
-            body32<MASK, RF>(jr[j], j64, j, i, I, a, sN, cN, eps2, eps2d, mscale);
+            body32<MASK, RF, TS, EQ>(jr[j], j64, j, i, I, a, sN, cN, eps2, eps2d, rscale);
             if (++cnt >= tile) {
                 cnt = 0;
                 fold_newton(a, sN, cN);
@@ -509,9 +529,9 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
     int j = j0;
     for (; j + 1 < j1; j += 2) {
         const JRec32 rb = jr[j + 1];
-        body32<MASK, RF>(ra, j64, j, i, I, a, sN, cN, eps2, eps2d, mscale);
+        body32<MASK, RF, TS, EQ>(ra, j64, j, i, I, a, sN, cN, eps2, eps2d, rscale);
         ra = jr[j + 2];
-        body32<MASK, RF>(rb, j64, j + 1, i, I, a, sN, cN, eps2, eps2d, mscale);
+        body32<MASK, RF, TS, EQ>(rb, j64, j + 1, i, I, a, sN, cN, eps2, eps2d, rscale);
         cnt += 2;
         if (cnt >= tile) {
             cnt = 0;
@@ -519,20 +539,19 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
         }
     }
     if (j < j1) {
-        body32<MASK, RF>(ra, j64, j, i, I, a, sN, cN, eps2, eps2d, mscale);
+        body32<MASK, RF, TS, EQ>(ra, j64, j, i, I, a, sN, cN, eps2, eps2d, rscale);
         ++cnt;
     }
 }
 
 
 // The Newtonian sums are Kahan-compensated every `tile` pairs (at least
-// 2): they carry the stars' accelerations, and plain FP32 sums over longer
-// runs are the largest error of this form after the close pairs (measured
-// at n = 131073: max per-star error 8e-7 compensating every 32 pairs,
-// 1.5e-6 every 256).  The other sums enter scaled by 1/c^2 (or, for the
+// 2): they carry the stars' accelerations, and with the close pairs taken
+// out of the FP32 run sums, a run of 32 pairs holds no term larger than
+// PN1_CLOSE_KAPPA |a|.  The other sums enter scaled by 1/c^2 (or, for the
 // BH, beside its FP64 Newtonian term) and accumulate in plain FP32 over
 // the whole chunk.
-template <int PP>
+template <int PP, bool TS, bool EQ>
 __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
                                               const JRec64* __restrict__ j64,
                                               const Thr* __restrict__ thr,
@@ -550,7 +569,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const I32 I = load_i32(jr, j64, thr, st, P.cap, i, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
-    const double eps2d = P.eps2, mscale = P.mmax;
+    const double eps2d = P.eps2, rscale = 1.0 / P.mmax;
     float a[NACC32], sumN[3], cmpN[3];
 #pragma unroll
     for (int k = 0; k < NACC32; ++k) a[k] = 0.0f;
@@ -558,11 +577,11 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     for (int k = 0; k < 3; ++k) sumN[k] = cmpN[k] = 0.0f;
     int cnt = 0;
     if (R.w0 < R.w1) {
-        run32<false, PP>(jr, j64, R.j0, R.w0, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
-        run32<true, PP>(jr, j64, R.w0, R.w1, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
-        run32<false, PP>(jr, j64, R.w1, R.j1, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
+        run32<false, PP, TS, EQ>(jr, j64, R.j0, R.w0, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
+        run32<true, PP, TS, EQ>(jr, j64, R.w0, R.w1, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
+        run32<false, PP, TS, EQ>(jr, j64, R.w1, R.j1, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
     } else {
-        run32<false, PP>(jr, j64, R.j0, R.j1, i, I, a, eps2, eps2d, mscale, sumN, cmpN, tile, cnt);
+        run32<false, PP, TS, EQ>(jr, j64, R.j0, R.j1, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
     }
     fold_newton(a, sumN, cmpN);
     if (li < n) {
@@ -570,7 +589,8 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
         const size_t cap = (size_t)P.cap;
         float* ps = part + (size_t)s * MROWS * cap + li;
 #pragma unroll
-        for (int k = 0; k < NACC32; ++k) ps[k * cap] = k <= A_NZ ? sumN[k] : a[k];
+        for (int k = 0; k < NACC32; ++k)
+            if (TS || k != A_TS) ps[k * cap] = k <= A_NZ ? sumN[k] : a[k];
 #pragma unroll
         for (int k = 0; k < 3; ++k) ps[(NACC32 + k) * cap] = cmpN[k];
     }
@@ -722,8 +742,8 @@ __device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
 
 // Stars p0 .. p1 - 1 (p0 = 1, p1 = n: every star; a sub-range is one i shard
 // of a plain evaluation, STAGE 0).
-// kout (mixed form): each star's cancellation ratio, for the next
-// evaluation's pair thresholds.
+// kout (mixed form, evaluations that accumulated A_TS; else null): each
+// star's cancellation ratio, for the next evaluations' pair thresholds.
 template <int MODE, int STAGE>
 __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double* Y, double* H1,
                                            double* H2, const double* __restrict__ m,
@@ -739,7 +759,7 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
         double S_[NACC], ts, a[3];
         gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb, a);
-        if (MODE == 1) kout[p] = cancel_ratio(ts, a);
+        if (MODE == 1 && kout) kout[p] = cancel_ratio(ts, a);
     }
     block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);
 }
