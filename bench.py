@@ -466,12 +466,17 @@ def main():
                     help="only measure the energy drift of both forms over this many steps "
                          "of the headline workload, print it as JSON and exit")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
-                    help="only run a plain copy kernel over MIB MiB, 3 times (for "
-                         "calibrating profiler byte counters), and exit")
+                    help="only copy MIB MiB between two device buffers with torch's copy "
+                         "kernel, 3 times (a workload of known HBM traffic for calibrating "
+                         "profiler byte counters), and exit")
     args = ap.parse_args()
     if args.calib_copy:
-        with pn.Context(capacity=1, device=0) as ctx:
-            ctx.util_copy(args.calib_copy << 20, reps=3)
+        import torch
+        src = torch.zeros(args.calib_copy << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes": args.calib_copy << 20, "reps": 3}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -143,11 +143,6 @@ int  pn1_autotune(pn1_ctx *ctx, int reps, int *jsplit, int *variant,
 int  pn1_profile_enable(pn1_ctx *ctx, int on);
 int  pn1_profile_read(pn1_ctx *ctx, int *launches, double *total_ms);
 
-/* Copies `bytes` between two scratch device buffers with a plain copy
- * kernel, `reps` times (allocates and frees them): a workload of known
- * HBM traffic for calibrating profiler byte counters. */
-int  pn1_util_copy(pn1_ctx *ctx, long long bytes, int reps);
-
 /* ---- adjoint (FP64) ---------------------------------------------------- */
 /* VJP of one force evaluation at the resident state: for a cotangent u
  * (AoS [n][3], one 3-vector per particle's acceleration) returns

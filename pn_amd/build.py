@@ -2,8 +2,7 @@
 
     python -m pn_amd.build          # or: from pn_amd.build import build; build()
 
-Outputs: pn_amd/libpn1.so, pn_amd/sticky9, pn_amd/valu_rates (objects in
-pn_amd/_build/).
+Outputs: pn_amd/libpn1.so and pn_amd/sticky9 (objects in pn_amd/_build/).
 Every HIP translation unit is compiled with packed FP32 disabled
 (-target-feature -packed-fp32-ops).
 """
@@ -19,7 +18,6 @@ BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libpn1.so")
 STICKY9 = os.path.join(HERE, "sticky9")
-VALU_RATES = os.path.join(HERE, "valu_rates")
 
 ARCH = "gfx950"
 HIP_FLAGS = [
@@ -64,7 +62,6 @@ def build(verbose=False):
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, adj_o, host_o, "-o", tmp],
          verbose)
     os.replace(tmp, LIB)
-    _run([hipcc, *HIP_FLAGS, os.path.join(CSRC, "valu_rates.hip"), "-o", VALU_RATES], verbose)
     sticky = os.path.join(CSRC, "sticky9.c")
     if os.path.exists(sticky):
         _run(["gcc", *C_FLAGS, "-I", INCLUDE, sticky, "-o", STICKY9, "-L", HERE, "-lpn1",

@@ -75,6 +75,15 @@ Phys make_phys(const pn1_ctx* c)
     return P;
 }
 
+// The mixed form's BH-close threshold for stars up to dmax from the BH (see
+// PN1_FLAG_ETA): tauf = eta / (6 u D) with D = 2 dmax, room for the stars to
+// move away before the next upload.
+double flag_threshold(double dmax)
+{
+    if (!(dmax > 0.0)) return HUGE_VAL;
+    return PN1_FLAG_ETA / (6.0 * 0x1p-24 * 2.0 * dmax);
+}
+
 // j split S and grid of the pair kernels for the current n.
 void plan(pn1_ctx* c)
 {
@@ -84,10 +93,10 @@ void plan(pn1_ctx* c)
     int S = c->jsplit_req;
     if (S <= 0) {
         // Many short rounds of workgroups: the last, partly filled round of
-        // a grid costs little when there are tens of rounds (pn_amd/tune.py
-        // on MI355X: at n = 131073 split 64 is 1.6x faster than split 4 for
-        // the mixed kernel; at n = 10241 split 128 is 1.2x faster than 32).
-        // Chunks stay >= 64 records.
+        // a grid costs little when there are tens of rounds (measured on
+        // MI355X at n = 131073: the mixed pair kernel takes 13.3 ms at split
+        // 64, 14.4 ms at 16 and 22.9 ms at 4; at n = 10241 split 128 is 1.2x
+        // faster than 32).  Chunks stay >= 64 records.
         S = 1;
         while (S < 128 && (long)S * c->nIblk < 32768 && (nst + 2 * S - 1) / (2 * S) >= 64) S *= 2;
     }
@@ -100,11 +109,11 @@ void plan(pn1_ctx* c)
     } else {
         c->pair_grid = S * c->nIblk;
     }
-    c->tile = c->tile_req > 0 ? (c->tile_req < 2 ? 2 : c->tile_req) : 32;
+    c->tile = c->tile_req > 0 ? (c->tile_req < 2 ? 2 : c->tile_req) : 64;
 }
 
-// Partial-sum buffer for a j split of S chunks (default: the plan's).
-int ensure_part(pn1_ctx* c, int split = 0)
+// Bytes of pair partials a j split of S chunks needs (0: the plan's).
+size_t part_need(const pn1_ctx* c, int split = 0)
 {
     const size_t cap = (size_t)c->cap, S = (size_t)(split > 0 ? split : c->S);
     size_t need = S * NACC * cap * sizeof(double);
@@ -112,15 +121,23 @@ int ensure_part(pn1_ctx* c, int split = 0)
     const size_t need32 = S * MROWS * cap * sizeof(float);
     if (needE > need) need = needE;
     if (need32 > need) need = need32;
+    return need;
+}
+
+// Partial-sum buffer for a j split of S chunks (default: the plan's).  The
+// new buffer is allocated before the old one is freed: if the allocation
+// is refused, the old buffer stays (ready() then refuses launches of a
+// plan it does not cover).
+int ensure_part(pn1_ctx* c, int split = 0)
+{
+    const size_t need = part_need(c, split);
     if (need <= c->part_bytes) return PN1_OK;
+    void* p = nullptr;
+    if (dev_malloc(&p, need) != hipSuccess)
+        return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes for pair partials failed", need);
     drop_graph(c);                      // a captured step holds the old buffer
     if (c->part) (void)hipFree(c->part);
-    c->part = nullptr;
-    c->part_bytes = 0;
-    if (dev_malloc(&c->part, need) != hipSuccess) {
-        c->part = nullptr;
-        return seterr(c, PN1_ENOMEM, "hipMalloc of %zu bytes for pair partials failed", need);
-    }
+    c->part = p;
     c->part_bytes = need;
     return PN1_OK;
 }
@@ -150,6 +167,12 @@ bool ready(pn1_ctx* c, int* rc)
         *rc = seterr(c, PN1_EINVAL, "no state uploaded");
         return false;
     }
+    if (c->part_bytes < part_need(c)) {
+        *rc = seterr(c, PN1_ENOMEM, "the pair partials (%zu bytes) do not cover the launch plan "
+                     "(%zu bytes): an earlier allocation was refused", c->part_bytes,
+                     part_need(c));
+        return false;
+    }
     if (hipSetDevice(c->dev) != hipSuccess) {
         *rc = seterr(c, PN1_EDEVICE, "hipSetDevice(%d) failed", c->dev);
         return false;
@@ -167,7 +190,7 @@ void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
     float* kout = (MODE == 1 && STAGE <= 1) ? c->kr : nullptr;
     hipLaunchKernelGGL((k_fin<MODE, STAGE>), dim3(c->nblk), dim3(WG), 0, c->st, in, c->X, c->Y,
                        c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk,
-                       kout, P, 1, c->n);
+                       kout, (const int*)c->fn, (const double*)c->xp, P, 1, c->n);
     hipLaunchKernelGGL((k_bh<STAGE>), dim3(1), dim3(WG), 0, c->st, c->bhblk, c->nblk, c->gs, c->X,
                        c->Y, c->H1, c->H2, c->acc, P);
 }
@@ -213,16 +236,56 @@ void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, 
     }
 }
 
+// k_prep's mixed-form inputs (prev: the thresholds from the previous
+// evaluation; false: none, the first pass after an upload).
+MixPrep mix_prep(const pn1_ctx* c, const Phys& P, bool prev)
+{
+    MixPrep M;
+    M.j32 = c->j32;
+    M.thr = c->thr;
+    M.aprev = prev ? c->acc : nullptr;
+    M.kprev = prev ? c->kr : nullptr;
+    M.mmax = c->mmax;
+    M.tauf = c->tauf;
+    M.lam = P.lam;
+    return M;
+}
+
+MixPrep fp64_prep()
+{
+    MixPrep M;
+    M.j32 = nullptr;
+    M.thr = nullptr;
+    M.aprev = nullptr;
+    M.kprev = nullptr;
+    M.mmax = M.tauf = M.lam = 0.0;
+    return M;
+}
+
+// The records and separable sums of one evaluation on `in`; mixed form:
+// also the list of stars taken out of the FP32 pass and the FP64 sums of
+// their pairs.
+void launch_prep(pn1_ctx* c, const double* in, const Phys& P)
+{
+    const bool mixed = c->mode == PN1_MIXED;
+    const MixPrep M = mixed ? mix_prep(c, P, c->acc_ok) : fp64_prep();
+    hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64, c->gblk, M,
+                       c->fl, c->fblk);
+    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
+    if (mixed) {
+        hipLaunchKernelGGL(k_flist, dim3(1), dim3(WG), 0, c->st, (const int*)c->fl,
+                           (const int*)c->fblk, c->nblk, c->n, c->flist, c->fn);
+        hipLaunchKernelGGL(k_pair_fl, dim3(c->nblk), dim3(WG), 0, c->st, (const JRec64*)c->j64,
+                           in, (const int*)c->flist, (const int*)c->fn, c->xp, P);
+    }
+}
+
 // stage 0: plain evaluation on X; stages 1..3: the driver's force calls.
 int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
 {
     const double* in = stage <= 1 ? c->X : c->Y;
     const bool mixed = c->mode == PN1_MIXED;
-    hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
-                       mixed ? c->j32 : nullptr, c->gblk,
-                       mixed && c->acc_ok ? (const double*)c->acc : nullptr,
-                       mixed && c->acc_ok ? (const float*)c->kr : nullptr, c->thr);
-    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
+    launch_prep(c, in, P);
     if (c->n > 1) {
         hipEvent_t e1 = nullptr;
         if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
@@ -247,7 +310,11 @@ int enqueue_force(pn1_ctx* c, int stage, const Phys& P)
 int ensure_prev_acc(pn1_ctx* c, const Phys& P)
 {
     if (c->mode != PN1_MIXED || c->acc_ok) return PN1_OK;
-    return enqueue_force(c, 0, P);
+    const bool prof = c->prof;      // not one of the caller's timed launches
+    c->prof = false;
+    const int rc = enqueue_force(c, 0, P);
+    c->prof = prof;
+    return rc;
 }
 
 int enqueue_step(pn1_ctx* c, const Phys& P)
@@ -302,6 +369,9 @@ void pn1_destroy(pn1_ctx* c)
     if (c->j32) (void)hipFree(c->j32);
     if (c->thr) (void)hipFree(c->thr);
     if (c->kr) (void)hipFree(c->kr);
+    void* fbufs[] = {c->fl, c->fblk, c->flist, c->fn, c->xp};
+    for (void* b : fbufs)
+        if (b) (void)hipFree(b);
     if (c->part) (void)hipFree(c->part);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -350,6 +420,11 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
     if (!rc) rc = dalloc(c, &c->j32, cap + 2);
     if (!rc) rc = dalloc(c, &c->thr, cap);
     if (!rc) rc = dalloc(c, &c->kr, cap);
+    if (!rc) rc = dalloc(c, &c->fl, cap);
+    if (!rc) rc = dalloc(c, &c->fblk, nb);
+    if (!rc) rc = dalloc(c, &c->flist, cap);
+    if (!rc) rc = dalloc(c, &c->fn, 4);
+    if (!rc) rc = dalloc(c, &c->xp, (size_t)(NACC + 1) * cap);
     if (!rc && host_malloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
         c->h_stage = nullptr;
         rc = PN1_ENOMEM;
@@ -384,6 +459,7 @@ int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile, int variant)
     if (jsplit < 0 || jsplit > 4096 || tile < 0 || tile > 65536 || variant < -1 || variant > 3)
         return seterr(c, PN1_EINVAL, "bad tuning (jsplit %d, tile %d, variant %d)", jsplit, tile,
                       variant);
+    const int keep_S = c->jsplit_req, keep_tile = c->tile_req, keep_var = c->variant_req;
     c->jsplit_req = jsplit;
     c->tile_req = tile;
     c->variant_req = variant;
@@ -392,7 +468,14 @@ int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile, int variant)
     HIPCHK(c, hipSetDevice(c->dev));
     HIPCHK(c, hipStreamSynchronize(c->st));
     plan(c);
-    return ensure_part(c);
+    const int rc = ensure_part(c);
+    if (rc) {                           // refused: the shape in use before stays
+        c->jsplit_req = keep_S;
+        c->tile_req = keep_tile;
+        c->variant_req = keep_var;
+        plan(c);
+    }
+    return rc;
 }
 
 int pn1_get_tuning(const pn1_ctx* c, int* jsplit, int* tile, int* variant)
@@ -433,10 +516,15 @@ int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, in
     c->m0 = m[0];
     c->mmax = 0.0;
     c->eqm = true;
+    double dmax2 = 0.0;
     for (int i = 1; i < n; ++i) {
         c->mmax = std::max(c->mmax, m[i]);
         c->eqm = c->eqm && m[i] == m[1];
+        double d2 = 0.0;
+        for (int k = 0; k < 3; ++k) d2 += (x[3 * i + k] - x[k]) * (x[3 * i + k] - x[k]);
+        dmax2 = std::max(dmax2, d2);
     }
+    c->tauf = flag_threshold(std::sqrt(dmax2));
     c->acc_ok = false;
     c->have_state = true;
     plan(c);
@@ -492,11 +580,7 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     if ((rc = ensure_part(c, S))) return rc;
     // every star's j-record and the separable sums: what a sharded run would
     // all-gather and all-reduce before its pair kernel
-    hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, in, c->m, P, c->j64,
-                       mixed ? c->j32 : nullptr, c->gblk,
-                       mixed ? (const double*)c->acc : nullptr,
-                       mixed ? (const float*)c->kr : nullptr, c->thr);
-    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
+    launch_prep(c, in, P);
     hipEvent_t e1 = nullptr;
     if (c->prof) HIPCHK(c, prof_open(c->ev, c->st, &e1));
     launch_pair(c, in, P, (i0 - 1) / WG, nib, S, true);
@@ -504,12 +588,12 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     const dim3 gf(nib);
     if (mixed)
         hipLaunchKernelGGL((k_fin<1, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr, P, i0,
-                           i1);
+                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr,
+                           (const int*)c->fn, (const double*)c->xp, P, i0, i1);
     else
         hipLaunchKernelGGL((k_fin<0, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr, P, i0,
-                           i1);
+                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr,
+                           (const int*)c->fn, (const double*)c->xp, P, i0, i1);
     if (!mixed) c->acc_ok = false;
     HIPCHK(c, hipGetLastError());
     const size_t cap = (size_t)c->cap, cnt = (size_t)(i1 - i0);
@@ -529,8 +613,7 @@ int pn1_energy(pn1_ctx* c, pn1_energy_t* out)
     if (!out) return seterr(c, PN1_EINVAL, "null output");
     const Phys P = make_phys(c);
     hipLaunchKernelGGL(k_prep, dim3(c->nblk), dim3(WG), 0, c->st, c->X, c->m, P, c->j64,
-                       (JRec32*)nullptr, c->gblk, (const double*)nullptr, (const float*)nullptr,
-                       (Thr*)nullptr);
+                       c->gblk, fp64_prep(), c->fl, c->fblk);
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     if (c->n > 1)
         hipLaunchKernelGGL(k_epair64, dim3(c->pair_grid), dim3(WG), 0, c->st, c->j64, c->X,
@@ -705,30 +788,6 @@ int pn1_autotune(pn1_ctx* c, int reps, int* jsplit, int* variant, double* best_m
     if (jsplit) *jsplit = c->S;
     if (variant) *variant = kernel_variant(c);
     if (best_ms) *best_ms = best;
-    return PN1_OK;
-}
-
-// Copies `bytes` (rounded down to 16) between two device buffers with
-// k_copy on the context's stream, `reps` times; for counter calibration.
-int pn1_util_copy(pn1_ctx* c, long long bytes, int reps)
-{
-    if (!c || bytes < 16 || reps < 1) return PN1_EINVAL;
-    HIPCHK(c, hipSetDevice(c->dev));
-    const size_t n2 = (size_t)bytes / 16;
-    void *a = nullptr, *b = nullptr;
-    if (dev_malloc(&a, n2 * 16) != hipSuccess) return seterr(c, PN1_ENOMEM, "copy buffer");
-    if (dev_malloc(&b, n2 * 16) != hipSuccess) {
-        (void)hipFree(a);
-        return seterr(c, PN1_ENOMEM, "copy buffer");
-    }
-    (void)hipMemsetAsync(a, 0, n2 * 16, c->st);
-    for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(k_copy, dim3(2048), dim3(WG), 0, c->st, (const double2*)a, (double2*)b,
-                           n2);
-    hipError_t e = hipStreamSynchronize(c->st);
-    (void)hipFree(a);
-    (void)hipFree(b);
-    if (e != hipSuccess) return seterr(c, PN1_EDEVICE, "copy: %s", hipGetErrorString(e));
     return PN1_OK;
 }
 

@@ -113,6 +113,31 @@ struct __align__(8) Thr {
     float rc2, dn2;
 };
 
+// Mixed form: stars close to the BH.  A star j at distance |d_j| from the
+// BH enters every other star's 1PN pair terms with factors e_j^3 ~ |d_j|^-3
+// (Limit-sticky9.c:302-308): 0.25 m0 e_j^3 (|r|^2 - |d_i|^2) and
+// 3.5 m0 m_j e_j^3 (w - e_i) c_j, each of which cancels (against the
+// zeta term and against e_i c_j) to a small result only after the sums.
+// In FP32 sums their rounding reaches ~6 u lam m_j e_j^2 |d_i| of a star
+// i's acceleration (|a_i| >= m0 / |d_i|^2).  A star with
+// lam m_j e_j^2 > tauf, tauf = PN1_FLAG_ETA / (6 u D), D twice the
+// farthest star from the BH at upload, is taken out of the FP32 pass (its
+// mixed record is neutralised: far away, zero mass, zero 1PN fields) and
+// its pairs with every star are evaluated in FP64 by k_pair_fl, whose sums
+// k_fin adds.  Such stars are rare (none in the benchmark's Plummer sphere
+// at N = 131072 within |d| < 3e-3).
+#define PN1_FLAG_ETA 1e-8
+#define PN1_FAR 1e18f
+// What k_prep needs for the mixed form besides the FP64 records.
+struct MixPrep {
+    JRec32* j32;            // null: FP64 form (nothing below is used)
+    Thr* thr;
+    const double* aprev;    // previous evaluation (both null after an upload)
+    const float* kprev;
+    double mmax, tauf;      // the cluster's largest star mass; BH-close threshold
+    double lam;
+};
+
 // ---------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------

@@ -22,9 +22,10 @@ struct pn1_ctx {
     double m0 = 0.0;            // host copy of m[0]
     double mmax = 0.0;          // largest star mass (mixed form's pair thresholds)
     bool eqm = false;           // every star has the same mass
+    double tauf = 0.0;          // BH-close threshold of the mixed form (PN1_FLAG_ETA)
     int jsplit_req = 0, tile_req = 0, variant_req = -1;
     // launch plan for the current n
-    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 32;
+    int S = 1, nIblk = 0, pair_grid = 0, nblk = 1, tile = 64;
     // device buffers
     double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr;
     double *m = nullptr, *acc = nullptr, *gblk = nullptr, *gs = nullptr;
@@ -33,6 +34,10 @@ struct pn1_ctx {
     pn1dev::JRec32* j32 = nullptr;
     pn1dev::Thr* thr = nullptr; // mixed form's per-star pair thresholds
     float* kr = nullptr;        // mixed form's per-star cancellation ratios
+    // mixed form: stars taken out of the FP32 pass (flags, per-block counts,
+    // their list and number) and the FP64 sums of their pairs
+    int *fl = nullptr, *fblk = nullptr, *flist = nullptr, *fn = nullptr;
+    double* xp = nullptr;
     // acc and kr hold the last mixed evaluation of the uploaded run (cleared
     // by pn1_upload and by FP64 evaluations): the mixed form's thresholds
     // read them

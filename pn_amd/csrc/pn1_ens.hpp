@@ -30,7 +30,7 @@ struct pn1_ens {
     double eps = 1e-3, c = 1.0, dt = 1.0 / 1024.0;
     int mode = PN1_FP64;
     int L_req = 0, tile_req = 0, variant_req = -1;
-    int L = 512, tile = 32;
+    int L = 512, tile = 64;
     double *X = nullptr, *Y = nullptr, *H1 = nullptr, *H2 = nullptr, *m = nullptr, *acc = nullptr;
     double *gblk = nullptr, *gs = nullptr, *bhblk = nullptr, *eblk = nullptr, *eout = nullptr;
     double* cm0 = nullptr;
@@ -39,6 +39,13 @@ struct pn1_ens {
     float* kr = nullptr;          // mixed form's per-star cancellation ratios
     bool acc_ok = false;          // acc, kr hold the last mixed evaluation
     bool eqm = false;             // each cluster's stars have one mass
+    double* ctauf = nullptr;      // each cluster's BH-close threshold (PN1_FLAG_ETA)
+    // mixed form: stars taken out of the FP32 pass (flags, per-block counts
+    // [nb_cap], lists at each cluster's offset, per-cluster numbers) and the
+    // FP64 sums of their pairs [NACC + 1][cap]
+    int *fl = nullptr, *fblk = nullptr, *flist = nullptr, *fn = nullptr;
+    double* xp = nullptr;
+    int fblk_cap = 0, gblk_cap = 0, bhblk_cap = 0, eblk_cap = 0;
     pn1dev::JRec64* j64 = nullptr;
     pn1dev::JRec32* j32 = nullptr;
     int *coff = nullptr, *cblk = nullptr, *qoff = nullptr;

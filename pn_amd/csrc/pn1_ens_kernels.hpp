@@ -102,7 +102,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
         if (it < 0) break;
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
-        const I32 I = load_i32(jr, j64, thr, st, P.cap, g.i, P.eps2);
+        const I32 I = load_i32(j64, thr, st, P.cap, g.i, P.eps2);
         if (!g.live) continue;                 // as in k_ens_pair64
         float a[NACC32], sN[3], cN[3];
 #pragma unroll
@@ -131,20 +131,19 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
     }
 }
 
-// aprev, kprev: the accelerations and cancellation ratios of the ensemble's
-// last mixed evaluation (null: none yet); cmmax: each cluster's largest
-// star mass (the mixed record's mass scale).
+// M: the mixed form's inputs (M.j32 null: FP64), with each cluster's largest
+// star mass cmmax[c] and BH-close threshold ctauf[c] in place of M.mmax,
+// M.tauf; fl, fblk: as k_prep.
 __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
                                                 const double* __restrict__ m,
                                                 const double* __restrict__ cm0,
                                                 const int* __restrict__ coff,
                                                 const EBlk* __restrict__ blk, Phys P,
                                                 JRec64* __restrict__ j64,
-                                                JRec32* __restrict__ j32,
-                                                double* __restrict__ gblk,
-                                                const double* aprev, const float* kprev,
+                                                double* __restrict__ gblk, MixPrep M,
                                                 const double* __restrict__ cmmax,
-                                                Thr* __restrict__ thr)
+                                                const double* __restrict__ ctauf,
+                                                int* __restrict__ fl, int* __restrict__ fblk)
 {
     const EBlk b = blk[blockIdx.x];
     const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
@@ -152,10 +151,52 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
     double g[NGS];
 #pragma unroll
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
-    if (li < nst)
-        prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, j32, g, aprev, kprev,
-                  j32 ? cmmax[b.c] : 0.0, thr);
+    if (M.j32) {
+        M.mmax = cmmax[b.c];
+        M.tauf = ctauf[b.c];
+    }
+    bool f = false;
+    if (li < nst) {
+        f = prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, g, M);
+        if (M.j32) fl[off + 1 + li] = f ? 1 : 0;
+    }
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
+    if (M.j32) {
+        const int cnt = __syncthreads_count(f);
+        if (threadIdx.x == 0) fblk[blockIdx.x] = cnt;
+    }
+}
+
+// One workgroup per cluster: its flagged stars, in increasing order, at
+// flist[coff[c] ..), their number in fn[c].
+__global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
+                                                 const int* __restrict__ fblk,
+                                                 const int* __restrict__ cblk,
+                                                 const int* __restrict__ coff,
+                                                 int* __restrict__ flist, int* __restrict__ fn)
+{
+    const int c = blockIdx.x;
+    list_flagged(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1, coff[c + 1], flist + coff[c],
+                 fn + c);
+}
+
+// Per star block: its stars' FP64 pairs with their cluster's flagged stars
+// (k_pair_fl per cluster), into xp [NACC + 1][cap] at the star's index.
+__global__ __launch_bounds__(WG) void k_ens_pair_fl(const JRec64* __restrict__ jr,
+                                                   const double* __restrict__ st,
+                                                   const int* __restrict__ coff,
+                                                   const EBlk* __restrict__ blk,
+                                                   const int* __restrict__ flist,
+                                                   const int* __restrict__ fn,
+                                                   double* __restrict__ xp, Phys P)
+{
+    const EBlk b = blk[blockIdx.x];
+    const int nf = fn[b.c];
+    const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
+    const int li = b.first + (int)threadIdx.x;
+    if (nf == 0 || li >= nst) return;
+    const int i = off + 1 + li;
+    flagged_pairs(jr, st, P.cap, i, P.eps2, flist + off, nf, xp, (size_t)P.cap, i);
 }
 
 // One workgroup per cluster: rows [cblk[c], cblk[c+1]) of `rows` (N wide).
@@ -196,7 +237,10 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
                                                double* __restrict__ acc,
                                                double* __restrict__ bhblk,
                                                const double* __restrict__ cmmax,
-                                               float* __restrict__ kout, Phys P, EConst E)
+                                               float* __restrict__ kout,
+                                               const int* __restrict__ fn,
+                                               const double* __restrict__ xp, Phys P,
+                                               EConst E)
 {
     const EBlk b = blk[blockIdx.x];
     const int off = coff[b.c], nst = coff[b.c + 1] - off - 1;
@@ -208,6 +252,7 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
                                      : (const void*)((const float*)part + poff[b.c] * MROWS);
         gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li,
                          MODE == 1 ? cmmax[b.c] : 1.0, S_, ts);
+        if (MODE == 1 && fn[b.c] > 0) add_flagged(xp, (size_t)P.cap, off + 1 + li, S_, ts);
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, off + 1 + li, off, S_,
                         gs + (size_t)b.c * NGS, fin_consts(P, cm0[b.c]), acc, cb, a);
         if (MODE == 1 && kout) kout[off + 1 + li] = cancel_ratio(ts, a);

@@ -91,17 +91,16 @@ __device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
 
 // Star p (SoA stride cap) of a cluster whose BH sits at index b0 and has
 // mass m0: writes its j-records and adds its terms of the separable sums
-// to g.  Mixed form (j32 set): also its pair thresholds, from its previous
-// acceleration aprev (SoA [3][cap]) and cancellation ratio kprev (both null
-// after an upload), and the cluster's largest star mass mmax (the mass
-// scale of the mixed record).
-__device__ __forceinline__ void prep_star(const double* __restrict__ st,
+// to g.  Mixed form (M.j32 set): also its pair thresholds, from its previous
+// acceleration M.aprev (SoA [3][cap]) and cancellation ratio M.kprev, and
+// its mixed record in units of the cluster's largest star mass M.mmax;
+// returns whether the star is close enough to the BH to be taken out of the
+// FP32 pass (see PN1_FLAG_ETA), in which case its mixed record is neutral.
+__device__ __forceinline__ bool prep_star(const double* __restrict__ st,
                                           const double* __restrict__ m, int cap, int p,
                                           int b0, double m0, double eps2,
-                                          JRec64* __restrict__ j64, JRec32* __restrict__ j32,
-                                          double (&g)[NGS], const double* aprev,
-                                          const float* kprev, double mmax,
-                                          Thr* __restrict__ thr)
+                                          JRec64* __restrict__ j64, double (&g)[NGS],
+                                          const MixPrep& M)
 {
 #pragma clang fp contract(off)
     const double x = st[p], y = st[cap + p], z = st[2 * cap + p];
@@ -123,42 +122,112 @@ __device__ __forceinline__ void prep_star(const double* __restrict__ st,
     r.Qdx = Q * dx; r.Qdy = Q * dy; r.Qdz = Q * dz;
     r.e = e; r.pad = 0.0;
     j64[p] = r;
-    if (j32) {
-        JRec32 h;
+    g[G_G1] = mp * e;
+    g[G_M1X] = q * dx; g[G_M1Y] = q * dy; g[G_M1Z] = q * dz;
+    g[G_PX] = mp * vx; g[G_PY] = mp * vy; g[G_PZ] = mp * vz;
+    if (!M.j32) return false;
+    const bool flagged = M.lam * mp * e * e > M.tauf;
+    JRec32 h;
+    if (flagged) {
+        // far away (w ~ 1e-18: every FP32 pair term vanishes), no mass and
+        // no 1PN fields (t = |r|^2 - D2_i is then ~1e36)
+        h.xh = h.yh = h.zh = PN1_FAR;
+        h.xl = h.yl = h.zl = 0.0f;
+        h.sx = h.sy = h.sz = 0.0f;
+        h.m = 0.0f;
+        h.A = h.B = h.MB = 0.0f;
+        h.Qdx = h.Qdy = h.Qdz = 0.0f;
+    } else {
         h.xh = (float)x; h.yh = (float)y; h.zh = (float)z;
         h.xl = (float)(x - (double)h.xh);
         h.yl = (float)(y - (double)h.yh);
         h.zl = (float)(z - (double)h.zh);
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
-        h.m = (float)(mp / mmax);
+        h.m = (float)(mp / M.mmax);
         h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
         h.Qdx = (float)r.Qdx; h.Qdy = (float)r.Qdy; h.Qdz = (float)r.Qdz;
-        j32[p] = h;
-        thr[p] = pair_thresholds(x, y, z, aprev, kprev, (size_t)cap, p, mmax);
     }
-    g[G_G1] = mp * e;
-    g[G_M1X] = q * dx; g[G_M1Y] = q * dy; g[G_M1Z] = q * dz;
-    g[G_PX] = mp * vx; g[G_PY] = mp * vy; g[G_PZ] = mp * vz;
+    M.j32[p] = h;
+    M.thr[p] = pair_thresholds(x, y, z, M.aprev, M.kprev, (size_t)cap, p, M.mmax);
+    return flagged;
 }
 
-// aprev, kprev: the accelerations and cancellation ratios of the context's
-// last mixed evaluation, read before this evaluation's k_fin overwrites
-// them (null: none yet).
+// fblk (mixed form): per block, the number of stars taken out of the FP32
+// pass; fl: per star, whether it is one of them.
 __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
                                             const double* __restrict__ m, Phys P,
                                             JRec64* __restrict__ j64,
-                                            JRec32* __restrict__ j32,
-                                            double* __restrict__ gblk,
-                                            const double* aprev, const float* kprev,
-                                            Thr* __restrict__ thr)
+                                            double* __restrict__ gblk, MixPrep M,
+                                            int* __restrict__ fl, int* __restrict__ fblk)
 {
     const int p = blockIdx.x * WG + threadIdx.x + 1;
     double g[NGS];
 #pragma unroll
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
-    if (p < P.n)
-        prep_star(st, m, P.cap, p, 0, P.m0, P.eps2, j64, j32, g, aprev, kprev, P.mmax, thr);
+    bool f = false;
+    if (p < P.n) {
+        f = prep_star(st, m, P.cap, p, 0, P.m0, P.eps2, j64, g, M);
+        if (M.j32) fl[p] = f ? 1 : 0;
+    }
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
+    if (M.j32) {
+        const int cnt = __syncthreads_count(f);
+        if (threadIdx.x == 0) fblk[blockIdx.x] = cnt;
+    }
+}
+
+// Exclusive prefix sum of one value per thread over the workgroup, in
+// thread order; *total receives the sum.
+__device__ __forceinline__ int block_excl_scan(int v, int* total)
+{
+    __shared__ int wsum[WG / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    int base = 0;
+#pragma unroll
+    for (int w = 0; w < WG / 64; ++w) base += w < wid ? wsum[w] : 0;
+    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return base + incl - v;
+}
+
+// The flagged stars of blocks [b0, b1) (block b covers the stars first + 0
+// .. first + WG - 1 of star index p0 + b * WG) listed in increasing order
+// at out[0 ..), their number in *nout.  One workgroup.
+__device__ __forceinline__ void list_flagged(const int* __restrict__ fl,
+                                             const int* __restrict__ fblk, int b0, int b1,
+                                             int p0, int pend, int* __restrict__ out,
+                                             int* __restrict__ nout)
+{
+    int base = 0;
+    for (int bb = b0; bb < b1; bb += WG) {
+        const int b = bb + (int)threadIdx.x;
+        const int cnt = b < b1 ? fblk[b] : 0;
+        int total;
+        const int at = base + block_excl_scan(cnt, &total);
+        if (cnt > 0) {
+            int k = at;
+            const int q0 = p0 + (b - b0) * WG;
+            for (int q = q0; q < q0 + WG && q < pend; ++q)
+                if (fl[q]) out[k++] = q;
+        }
+        base += total;
+    }
+    if (threadIdx.x == 0) *nout = base;
+}
+
+__global__ __launch_bounds__(WG) void k_flist(const int* __restrict__ fl,
+                                             const int* __restrict__ fblk, int nblk, int n,
+                                             int* __restrict__ flist, int* __restrict__ fn)
+{
+    list_flagged(fl, fblk, 0, nblk, 1, n, flist, fn);
 }
 
 // Fixed-order sum of nblk rows of N doubles (one workgroup).
@@ -313,21 +382,23 @@ struct I32 {
     float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2e, rc2, dn2;
 };
 
-// Star i: its mixed j-record, velocity, |d_i|^2 (from its FP64 record) and
+// Star i: its double-single position, velocity, |d_i|^2 (from its FP64
+// record: its mixed record is neutral if the star is close to the BH) and
 // pair thresholds.
-__device__ __forceinline__ I32 load_i32(const JRec32* __restrict__ jr,
-                                        const JRec64* __restrict__ j64,
+__device__ __forceinline__ I32 load_i32(const JRec64* __restrict__ j64,
                                         const Thr* __restrict__ thr,
                                         const double* __restrict__ st, int cap, int i,
                                         double eps2)
 {
-    const JRec32& r = jr[i];
+    const JRec64& r = j64[i];
     const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
-    const double D2 = j64[i].D2;
+    const double D2 = r.D2;
     const Thr t = thr[i];
     I32 I;
-    I.xh = r.xh; I.yh = r.yh; I.zh = r.zh;
-    I.xl = r.xl; I.yl = r.yl; I.zl = r.zl;
+    I.xh = (float)r.x; I.yh = (float)r.y; I.zh = (float)r.z;
+    I.xl = (float)(r.x - (double)I.xh);
+    I.yl = (float)(r.y - (double)I.yh);
+    I.zl = (float)(r.z - (double)I.zh);
     I.gx = (float)(4.0 * vx); I.gy = (float)(4.0 * vy); I.gz = (float)(4.0 * vz);
     I.hx = (float)(PN1_4OSQ15 * vx); I.hy = (float)(PN1_4OSQ15 * vy);
     I.hz = (float)(PN1_4OSQ15 * vz);
@@ -375,9 +446,9 @@ __device__ __forceinline__ void two_sum_into(float s, float y, float& u, float& 
 // compensation).  Every lane computes; the other lanes keep their sums.
 // The branch around it is wave-uniform, so j and the record stream stay
 // scalar.
-__device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const I32& I,
-                                            double eps2, double rscale, float mw3, float rx,
-                                            float ry, float rz, float (&a)[NACC32],
+__device__ __forceinline__ void close_pairs(bool close, const JRec64& J, float jm32,
+                                            const I32& I, double eps2, double rscale, float mw3,
+                                            float rx, float ry, float rz, float (&a)[NACC32],
                                             float (&sN)[3], float (&cN)[3])
 {
 #pragma clang fp contract(off)
@@ -392,7 +463,9 @@ __device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const I
     const double dz = J.z - ((double)ph[2] + (double)pl[2]);
     const double R2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
     const double w = rsq64(R2);
-    const double mw3d = (J.m * rscale) * (w * (w * w));   // the record's mass units
+    // in the record's mass units; none for a star taken out of the FP32 pass
+    // (its mixed record, mass jm32, is neutral)
+    const double mw3d = (jm32 != 0.0f ? J.m * rscale : 0.0) * (w * (w * w));
     const double d[3] = {dx, dy, dz};
     const float r[3] = {rx, ry, rz};
 #pragma unroll
@@ -464,7 +537,7 @@ __device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict
         if (__builtin_amdgcn_ballot_w64(close)) {
             const float wc = rsq32<RF>(R2e);
             const float mw3c = (EQ && !MASK ? wc : mj * wc) * (wc * wc);
-            close_pairs(close, j64[j], I, eps2d, rscale, mw3c, rx, ry, rz, a, sN, cN);
+            close_pairs(close, j64[j], J.m, I, eps2d, rscale, mw3c, rx, ry, rz, a, sN, cN);
         }
     }
     const float w = rsq32<RF>(R2e);
@@ -546,9 +619,9 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
 
 
 // The Newtonian sums are Kahan-compensated every `tile` pairs (at least
-// 2): they carry the stars' accelerations, and with the close pairs taken
-// out of the FP32 run sums, a run of 32 pairs holds no term larger than
-// PN1_CLOSE_KAPPA |a|.  The other sums enter scaled by 1/c^2 (or, for the
+// 2; 64 by default): they carry the stars' accelerations, and with the
+// close pairs taken out of the FP32 run sums, a run holds no term larger
+// than PN1_CLOSE_KAPPA |a|.  The other sums enter scaled by 1/c^2 (or, for the
 // BH, beside its FP64 Newtonian term) and accumulate in plain FP32 over
 // the whole chunk.
 template <int PP, bool TS, bool EQ>
@@ -566,7 +639,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(jr, j64, thr, st, P.cap, i, P.eps2);
+    const I32 I = load_i32(j64, thr, st, P.cap, i, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
     const double eps2d = P.eps2, rscale = 1.0 / P.mmax;
@@ -599,6 +672,45 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
 // ---------------------------------------------------------------------
 // k_fin: star accelerations, stage update, BH partials
 // ---------------------------------------------------------------------
+// The pairs of every star i with the stars taken out of the FP32 pass
+// (flist[0 .. nf), see PN1_FLAG_ETA), in FP64 with the FP64 form's body:
+// per star the NACC sums and sum m_j w^2, written to xp [NACC + 1][stride]
+// at idx.  Nothing when nf is 0.
+__device__ __forceinline__ void flagged_pairs(const JRec64* __restrict__ jr,
+                                              const double* __restrict__ st, int cap, int i,
+                                              double eps2, const int* __restrict__ flist,
+                                              int nf, double* __restrict__ xp, size_t stride,
+                                              int idx)
+{
+    const I64 I = load_i64(jr, st, cap, i, eps2);
+    double a[NACC], ts = 0.0;
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+    for (int f = 0; f < nf; ++f) {
+        const int j = flist[f];
+        body64<true>(jr[j], j, i, I, a, eps2);
+        if (j != i) {
+            const double rx = jr[j].x - I.x, ry = jr[j].y - I.y, rz = jr[j].z - I.z;
+            ts += jr[j].m / (rx * rx + ry * ry + rz * rz + eps2);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) xp[(size_t)k * stride + idx] = a[k];
+    xp[(size_t)NACC * stride + idx] = ts;
+}
+
+__global__ __launch_bounds__(WG) void k_pair_fl(const JRec64* __restrict__ jr,
+                                               const double* __restrict__ st,
+                                               const int* __restrict__ flist,
+                                               const int* __restrict__ fn,
+                                               double* __restrict__ xp, Phys P)
+{
+    const int nf = *fn;
+    const int i = blockIdx.x * WG + threadIdx.x + 1;
+    if (nf == 0 || i >= P.n) return;
+    flagged_pairs(jr, st, P.cap, i, P.eps2, flist, nf, xp, (size_t)P.cap, i);
+}
+
 // Sum of the S chunk partials of one star.  `part` points at the
 // cluster's partial area, `stride` is its per-sum stride and `idx` the
 // star's index in it.  FP64: [S][NACC][stride]; mixed: [S][MROWS][stride]
@@ -632,6 +744,16 @@ __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S,
             if (k <= A_UZ || k == A_KS || k == A_PHI || k == A_W3) A[k] *= mscale;
         ts *= mscale;
     }
+}
+
+// Adds one star's FP64 sums of its pairs with the stars taken out of the
+// FP32 pass (k_pair_fl) to its gathered sums.
+__device__ __forceinline__ void add_flagged(const double* __restrict__ xp, size_t stride, int idx,
+                                            double (&A)[NACC], double& ts)
+{
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) A[k] += xp[(size_t)k * stride + idx];
+    ts += xp[(size_t)NACC * stride + idx];
 }
 
 // Mixed form: the star's cancellation ratio for the next evaluation's
@@ -744,6 +866,8 @@ __device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
 // of a plain evaluation, STAGE 0).
 // kout (mixed form, evaluations that accumulated A_TS; else null): each
 // star's cancellation ratio, for the next evaluations' pair thresholds.
+// Mixed form: plus the FP64 sums xp of the pairs with the stars taken out of
+// the FP32 pass (when *fn > 0).
 template <int MODE, int STAGE>
 __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double* Y, double* H1,
                                            double* H2, const double* __restrict__ m,
@@ -751,13 +875,16 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
                                            const double* __restrict__ gs,
                                            double* __restrict__ acc,
                                            double* __restrict__ bhblk, float* __restrict__ kout,
-                                           Phys P, int p0, int p1)
+                                           const int* __restrict__ fn,
+                                           const double* __restrict__ xp, Phys P, int p0,
+                                           int p1)
 {
     const int p = p0 + blockIdx.x * WG + threadIdx.x;
     double cb[3] = {0.0, 0.0, 0.0};
     if (p < p1) {
         double S_[NACC], ts, a[3];
         gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
+        if (MODE == 1 && *fn > 0) add_flagged(xp, (size_t)P.cap, p, S_, ts);
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb, a);
         if (MODE == 1 && kout) kout[p] = cancel_ratio(ts, a);
     }
@@ -949,15 +1076,6 @@ __global__ __launch_bounds__(WG) void k_ered(const double* __restrict__ eblk, in
         out[3] = pn2;
         out[4] = kin + pot + pn + pn2;
     }
-}
-
-// Plain streaming copy (16 B per lane per iteration): the reference
-// workload for calibrating rocprof's HBM byte counters.
-__global__ __launch_bounds__(WG) void k_copy(const double2* __restrict__ src,
-                                            double2* __restrict__ dst, size_t n2)
-{
-    const size_t stride = (size_t)gridDim.x * WG;
-    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n2; i += stride) dst[i] = src[i];
 }
 
 }  // namespace pn1dev

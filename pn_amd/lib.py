@@ -69,7 +69,6 @@ def load():
         "pn1_get_tuning": (_I, [P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
         "pn1_profile_enable": (_I, [P, _I]),
         "pn1_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
-        "pn1_util_copy": (_I, [P, ctypes.c_longlong, _I]),
         "pn1_autotune": (_I, [P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _D]),
         "pn1_ens_autotune": (_I, [P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _D]),
         "pn1_force_vjp": (_I, [P, _D, _D, _D, _D]),
@@ -217,9 +216,6 @@ class Context:
 
     def profile_enable(self, on=True):
         self._chk(self._L.pn1_profile_enable(self._h, 1 if on else 0), "pn1_profile_enable")
-
-    def util_copy(self, nbytes, reps=1):
-        self._chk(self._L.pn1_util_copy(self._h, int(nbytes), int(reps)), "pn1_util_copy")
 
     def profile_read(self):
         k, ms = _I(), ctypes.c_double()

@@ -171,7 +171,9 @@ def test_mixed_accuracy(ens, clusters):
         if len(clusters[k][2]) < 2:
             continue
         err = rel_err(a32[k], a64[k])
-        assert err[1:].max() <= 1e-5 and err[0] <= 1e-6, (k, err[1:].max(), err[0])
+        print(f"cluster {k} n={len(clusters[k][2])}: mixed star max {err[1:].max():.3e} "
+              f"BH {err[0]:.3e}")
+        assert err[1:].max() <= 1e-6 and err[0] <= 1e-6, (k, err[1:].max(), err[0])
 
 
 def test_graph_replay_equals_eager_profiled(ens, clusters):

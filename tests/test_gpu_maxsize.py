@@ -18,8 +18,9 @@ Tolerances:
     worst row of the sample on MI355X: 2.9e-13 at n = 270000, c = 1 (a star
     near the BH, whose 1PN terms cancel against each other), 3.6e-14 at
     c = 50, 2.7e-14 at the headline size.
-  Mixed: against FP64, the headline bound: p99.9 <= 1e-6, max <= 1e-5
-    (stars), BH <= 1e-6.
+  Mixed: against FP64 (every star) and the oracle rows: max <= 1e-6,
+    BH <= 1e-6, at the initial state and one FP64 step on.  Measured on
+    MI355X: 3.9e-7 (NMAX, one step on).
 """
 import numpy as np
 import pytest
@@ -56,9 +57,16 @@ def mixed_vs_fp64(a32, a64, label):
     err = rel_err(a32, a64)
     print(f"{label}: mixed star p99.9 {np.quantile(err[1:], 0.999):.3e} max {err[1:].max():.3e} "
           f"BH {err[0]:.3e}", flush=True)
-    assert np.quantile(err[1:], 0.999) <= 1e-6
-    assert err[1:].max() <= 1e-5
+    assert err[1:].max() <= 1e-6
     assert err[0] <= 1e-6
+
+
+def mixed_rows_vs_oracle(a32, x, v, m, eps, c, seed, label):
+    rows = sample_rows(x, seed)
+    ref = ob.orc_force_rows(x, v, m, eps, c, rows)
+    err = np.linalg.norm(a32[rows] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"{label}: mixed vs {len(rows)} oracle rows max rel {err.max():.3e}", flush=True)
+    assert err.max() <= 1e-6, (rows[err.argmax()], err.max())
 
 
 @pytest.fixture(scope="module")
@@ -83,13 +91,26 @@ def test_nmax_fp64_rows_vs_oracle(nmax, c):
 
 def test_nmax_mixed_and_momentum(nmax):
     ctx, x, v, m = nmax
-    print(f"case: n = {NMAX} Plummer, mixed vs FP64, c = 50", flush=True)
+    print(f"case: n = {NMAX} Plummer, mixed vs FP64 and oracle rows, c = 50", flush=True)
     ctx.set_params(1e-4, 50.0, 2.0**-12, pn.FP64)
     ctx.upload(x, v, m)
     a64 = ctx.force()
     ctx.set_params(1e-4, 50.0, 2.0**-12, pn.MIXED)
     a32 = ctx.force()
     mixed_vs_fp64(a32, a64, f"n={NMAX} c=50")
+    mixed_rows_vs_oracle(a32, x, v, m, 1e-4, 50.0, 11, f"n={NMAX} c=50")
+    # the same after one FP64 step (positions with all 53 bits: the
+    # double-single lo parts are then not zero)
+    ctx.set_params(1e-4, 50.0, 2.0**-12, pn.FP64)
+    ctx.step(1)
+    xs, vs = ctx.download()
+    a64 = ctx.force()
+    ctx.set_params(1e-4, 50.0, 2.0**-12, pn.MIXED)
+    ctx.upload(xs, vs, m)
+    a32 = ctx.force()
+    mixed_vs_fp64(a32, a64, f"n={NMAX} c=50, one step on")
+    mixed_rows_vs_oracle(a32, xs, vs, m, 1e-4, 50.0, 12, f"n={NMAX} c=50, one step on")
+    ctx.upload(x, v, m)
     # Newtonian part alone (c -> infinity), FP64: the total momentum change
     # vanishes, BH row included
     ctx.set_params(1e-4, 1e30, 2.0**-12, pn.FP64)

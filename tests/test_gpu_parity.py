@@ -138,8 +138,13 @@ def test_mixed_vs_fp64(ctx, name):
         stars = err[1:]
         print(f"{name} c={c}: mixed star p99.9 {np.quantile(stars, 0.999):.3e} "
               f"max {stars.max():.3e} BH {err[0]:.3e}")
-        assert np.quantile(stars, 0.999) <= (5e-6 if c == 1 else 1e-6)
-        assert stars.max() <= 1e-5
+        if c >= 50:
+            assert stars.max() <= 1e-6
+        else:
+            # c = 1: the 1PN terms are as large as the Newtonian ones and
+            # their pair sums are FP32 (the target is stated for c >= 50)
+            assert np.quantile(stars, 0.999) <= 5e-6
+            assert stars.max() <= 1e-5
         assert err[0] <= 1e-6
 
 
