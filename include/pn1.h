@@ -108,6 +108,30 @@ int  pn1_force(pn1_ctx *ctx, double *a);
  * for its smaller grid) and agrees to rounding.  No counterpart in the
  * reference, which runs on one processor. */
 int  pn1_force_range(pn1_ctx *ctx, int i0, int i1, double *a);
+/* Largest n that pn1_force_noalloc takes. */
+#define PN1_NOALLOC_MAX_N 16384
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the FP64 force of pn1_force on a state
+ * that already lies in device memory.  It allocates nothing: every
+ * intermediate lives in a fixed scratch area of the library, 69 MB on each
+ * GPU it runs on, held from the library's load there until the process
+ * ends.
+ *   st      device, 6 rows of n doubles: x, y, z, vx, vy, vz (SoA)
+ *   m       device, n masses (m[0] the BH)
+ *   acc     device, out: 3 rows of n doubles, ax, ay, az (SoA)
+ *   stream  the hipStream_t to enqueue on (NULL: the null stream) of the
+ *           current device; the call returns once the kernels are enqueued
+ * Requires 1 <= n <= PN1_NOALLOC_MAX_N, finite eps > 0 and c > 0 (else
+ * PN1_EINVAL, nothing enqueued); the state itself is not checked.  The j
+ * split is 32 chunks, halved while a chunk would hold fewer than 64 stars:
+ * pn1_force with that split set (pn1_set_tuning) gives the same bits.  All
+ * calls on one GPU share its scratch, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams or threads) race,
+ * and the caller serialises them.  The first call on a device looks up the
+ * scratch's address; make it before capturing a graph. */
+int  pn1_force_noalloc(const double *st, const double *m, int n, double eps,
+                       double c, double *acc, void *stream);
 /* Energy of the resident state. */
 int  pn1_energy(pn1_ctx *ctx, pn1_energy_t *out);
 /* `steps` steps of the reference's three-evaluation stage scheme

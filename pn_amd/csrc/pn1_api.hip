@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -339,9 +340,64 @@ int fetch3(pn1_ctx* c, const double* dsrc, double* out)
     return PN1_OK;
 }
 
+// ---- scratch of pn1_force_noalloc -----------------------------------------
+// Sized for n <= PN1_NOALLOC_MAX_N and a j split of at most NA_MAXS chunks:
+// j-records, per-block separable sums, their totals, pair partials and the
+// per-block BH partials of one FP64 force evaluation, each part 256-byte
+// aligned (offsets in doubles).
+constexpr int NA_MAXS = 32;
+constexpr int NA_MAXB = (PN1_NOALLOC_MAX_N + WG - 1) / WG;
+constexpr size_t na_round(size_t d) { return (d + 31) & ~(size_t)31; }
+constexpr size_t NA_J64 = 0;
+constexpr size_t NA_PART =
+    NA_J64 + na_round((size_t)PN1_NOALLOC_MAX_N * (sizeof(JRec64) / sizeof(double)));
+constexpr size_t NA_GBLK = NA_PART + na_round((size_t)NA_MAXS * NACC * PN1_NOALLOC_MAX_N);
+constexpr size_t NA_GS = NA_GBLK + na_round((size_t)NA_MAXB * NGS);
+constexpr size_t NA_BH = NA_GS + na_round(NGS);
+constexpr size_t NA_TOTAL = NA_BH + na_round((size_t)NA_MAXB * 3);
+static_assert(sizeof(JRec64) % sizeof(double) == 0, "JRec64 in doubles");
+
+// j split of pn1_force_noalloc: NA_MAXS chunks, halved while a chunk would
+// hold fewer than 64 stars.
+int na_split(int nst)
+{
+    int S = NA_MAXS;
+    while (S > 1 && (nst + S - 1) / S < 64) S >>= 1;
+    return S;
+}
+
 }  // namespace pn1host
 
 using namespace pn1host;
+
+// One per GPU: the code object's __device__ array, placed on each device the
+// library runs on when the code object is loaded there, and held until the
+// process ends.  Host code never passes its name to a kernel or a copy (on
+// the host the name is only a handle): it uses the device address that
+// hipGetSymbolAddress returns for the current device (na_scratch).
+static __device__ __align__(256) double scratch[NA_TOTAL];
+
+namespace pn1host {
+
+constexpr int NA_MAXDEV = 64;
+double* g_na_ptr[NA_MAXDEV];
+std::mutex g_na_mu;
+
+// Device address of the current GPU's scratch, looked up once per device.
+int na_scratch(int dev, double** out)
+{
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (dev < 0 || dev >= NA_MAXDEV) return PN1_ENODEV;
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PN1_EDEVICE;
+        g_na_ptr[dev] = (double*)p;
+    }
+    *out = g_na_ptr[dev];
+    return PN1_OK;
+}
+
+}  // namespace pn1host
 
 // =========================================================================
 // context API
@@ -789,6 +845,64 @@ int pn1_autotune(pn1_ctx* c, int reps, int* jsplit, int* variant, double* best_m
     if (variant) *variant = kernel_variant(c);
     if (best_ms) *best_ms = best;
     return PN1_OK;
+}
+
+// =========================================================================
+// no-allocation force
+// =========================================================================
+// The FP64 force of pn1_force on a state the caller holds in device memory,
+// with every intermediate in this GPU's scratch: the same kernels, the j
+// split na_split(n - 1).  Enqueues only (no hipMalloc, hipFree, memset or
+// synchronisation); nothing in the scratch needs clearing, every value read
+// is written earlier in the same call.
+int pn1_force_noalloc(const double* st, const double* m, int n, double eps, double c,
+                      double* acc, void* stream)
+{
+    if (!st || !m || !acc || n < 1 || n > PN1_NOALLOC_MAX_N) return PN1_EINVAL;
+    if (!(eps > 0.0) || !std::isfinite(eps) || !(c > 0.0) || !std::isfinite(c))
+        return PN1_EINVAL;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return PN1_EDEVICE;
+    double* base = nullptr;
+    const int rc = na_scratch(dev, &base);
+    if (rc) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    Phys P;
+    P.eps = eps;
+    P.eps2 = eps * eps;
+    P.lam = 1.0 / (c * c);
+    P.dt = 0.0;
+    P.m0 = 0.0;                 // the kernels read the BH mass from m[0]
+    P.ws = 1.0 / std::sqrt(P.eps2);
+    P.mmax = 0.0;
+    P.eps2f = (float)P.eps2;
+    P.m0f = 0.0f;
+    P.n = n;
+    P.cap = n;
+    JRec64* j64 = (JRec64*)(base + NA_J64);
+    double* part = base + NA_PART;
+    double* gblk = base + NA_GBLK;
+    double* gs = base + NA_GS;
+    double* bhblk = base + NA_BH;
+    const int nst = n - 1, nblk = nblocks(nst), nib = (nst + WG - 1) / WG;
+    const int S = na_split(nst);
+    hipLaunchKernelGGL(k_prep, dim3(nblk), dim3(WG), 0, s, st, m, P, j64, gblk, fp64_prep(),
+                       (int*)nullptr, (int*)nullptr);
+    hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, s, (const double*)gblk, nblk, gs);
+    if (nst > 0) {
+        int grid = S * nib;
+        if (S <= 8 && 8 % S == 0) grid = 8 * ((nib + 8 / S - 1) / (8 / S));
+        hipLaunchKernelGGL(k_pair64<0>, dim3(grid), dim3(WG), 0, s, (const JRec64*)j64, st, part,
+                           P, S, nib, 0);
+    }
+    hipLaunchKernelGGL((k_fin<0, 0>), dim3(nblk), dim3(WG), 0, s, st, (double*)nullptr,
+                       (double*)nullptr, (double*)nullptr, (double*)nullptr, m,
+                       (const void*)part, S, (const double*)gs, acc, bhblk, (float*)nullptr,
+                       (const int*)nullptr, (const double*)nullptr, P, 1, n);
+    hipLaunchKernelGGL((k_bh<0>), dim3(1), dim3(WG), 0, s, (const double*)bhblk, nblk,
+                       (const double*)gs, (double*)nullptr, (double*)nullptr, (double*)nullptr,
+                       (double*)nullptr, acc, P);
+    return hipGetLastError() == hipSuccess ? PN1_OK : PN1_EDEVICE;
 }
 
 // =========================================================================

@@ -153,7 +153,7 @@ __device__ __forceinline__ bool prep_star(const double* __restrict__ st,
 }
 
 // fblk (mixed form): per block, the number of stars taken out of the FP32
-// pass; fl: per star, whether it is one of them.
+// pass; fl: per star, whether it is one of them.  The BH mass is m[0].
 __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
                                             const double* __restrict__ m, Phys P,
                                             JRec64* __restrict__ j64,
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
     bool f = false;
     if (p < P.n) {
-        f = prep_star(st, m, P.cap, p, 0, P.m0, P.eps2, j64, g, M);
+        f = prep_star(st, m, P.cap, p, 0, m[0], P.eps2, j64, g, M);
         if (M.j32) fl[p] = f ? 1 : 0;
     }
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
@@ -863,7 +863,7 @@ __device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
 }
 
 // Stars p0 .. p1 - 1 (p0 = 1, p1 = n: every star; a sub-range is one i shard
-// of a plain evaluation, STAGE 0).
+// of a plain evaluation, STAGE 0); the BH mass is m[0].
 // kout (mixed form, evaluations that accumulated A_TS; else null): each
 // star's cancellation ratio, for the next evaluations' pair thresholds.
 // Mixed form: plus the FP64 sums xp of the pairs with the stars taken out of
@@ -885,7 +885,7 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
         double S_[NACC], ts, a[3];
         gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
         if (MODE == 1 && *fn > 0) add_flagged(xp, (size_t)P.cap, p, S_, ts);
-        fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, P.m0), acc, cb, a);
+        fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, m[0]), acc, cb, a);
         if (MODE == 1 && kout) kout[p] = cancel_ratio(ts, a);
     }
     block_sum<3>(cb, bhblk + (size_t)blockIdx.x * 3);

@@ -70,6 +70,8 @@ def load():
         "pn1_profile_enable": (_I, [P, _I]),
         "pn1_profile_read": (_I, [P, ctypes.POINTER(_I), _D]),
         "pn1_autotune": (_I, [P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _D]),
+        "pn1_force_noalloc": (_I, [ctypes.c_void_p, ctypes.c_void_p, _I, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]),
         "pn1_ens_autotune": (_I, [P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _D]),
         "pn1_force_vjp": (_I, [P, _D, _D, _D, _D]),
         "pn1_adjoint_forward": (_I, [P, _I, _I]),
@@ -392,6 +394,28 @@ class Ensemble:
         self._chk(self._L.pn1_ens_adjoint_backward(self._h, ptr(gx), ptr(gv), ptr(gx0), ptr(gv0),
                                                    ptr(gm0)), "pn1_ens_adjoint_backward")
         return self.split(gx0), self.split(gv0), gm0
+
+
+# ---------------------------------------------------------------- no-allocation force
+NOALLOC_MAX_N = 16384
+
+
+def noalloc_split(n):
+    """The j split pn1_force_noalloc uses for n particles."""
+    nst, s = n - 1, 32
+    while s > 1 and (nst + s - 1) // s < 64:
+        s //= 2
+    return s
+
+
+def force_noalloc(st, m, n, eps, c, acc, stream=0):
+    """pn1_force_noalloc on device addresses (ints): st 6 x n doubles, m n
+    doubles, acc 3 x n doubles out; enqueued on `stream` (a hipStream_t
+    address, 0 for the null stream)."""
+    rc = load().pn1_force_noalloc(ctypes.c_void_p(st), ctypes.c_void_p(m), int(n), float(eps),
+                                  float(c), ctypes.c_void_p(acc), ctypes.c_void_p(stream))
+    if rc != PN1_OK:
+        raise Pn1Error(rc, f"pn1_force_noalloc(n={n}, eps={eps}, c={c}) failed")
 
 
 # ---------------------------------------------------------------- legacy shapes
