@@ -13,10 +13,20 @@ Prints ONE JSON line.  `value` is the mixed-precision form's interactions/s
 (star-star pairs in FP32 on double-single positions, everything else FP64);
 the FP64 form is measured the same way and reported beside it, with the
 per-particle acceleration error of the mixed form against the FP64 form at
-the same state.  The pair kernel's average launch duration comes from HIP
-events recorded around every pair launch on the library's stream during the
-timed steps.  Algorithmic work: 90 flops per interaction (FMA = 2,
+the same state (the initial state and two FP64 steps on).  The timed steps
+replay the captured step graph, as pn1_step runs them; the pair kernel's
+average launch duration comes from a separate pass of 3 steps after them,
+launched one by one with HIP events around every pair launch on the
+library's stream.  Algorithmic work: 90 flops per interaction (FMA = 2,
 rsqrt = 1; Newtonian 20 + three-body 1PN 70).
+
+CPU legs (skipped with --no-cpu), on the host's cores beside the GPU legs:
+the reference's own force() (compiled from its sources into oracle/_ref)
+on one core for the first 24576 stars (the bounded sample, `cpu_baseline`)
+and at n = 4097, 16385, 65537 for an n^2 fit and its projection to the full
+n; the oracle restatement over OpenMP threads (`cpu_openmp`); and, as the
+checker, the oracle's force rows at a sample of stars against both GPU
+forms (`accuracy_vs_oracle_rows`).
 
 The line also carries an "ensemble" object: the batched workload of 256
 seeded Plummer clusters (star counts log-uniform in [256, 8192]) advanced
@@ -53,6 +63,7 @@ FLOPS_PER_INTERACTION = 90
 PEAK_FP32 = 157.3e12   # MI355X vector FP32
 PEAK_FP64 = 78.6e12    # MI355X vector FP64
 CPU_SAMPLE_STARS = 24576
+CPU_FIT_SIZES = (4097, 16385, 65537)
 CPU_OMP_SAMPLE_STARS = 49152
 ENS_TRAFFIC = "ensemble_pair_traffic_c5.json"
 
@@ -61,23 +72,28 @@ def rel_err(a, ref):
     return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
 
 
-def run_form(ctx, x, v, m, mode, steps, warmup, autotune=True):
+def run_form(ctx, x, v, m, mode, steps, warmup, autotune=True, prof_steps=3):
+    """`steps` timed steps (graph-replayed, as pn1_step runs them) after
+    `warmup`; then, outside the timed region, `prof_steps` steps launched
+    one by one with HIP events around every pair launch on the library's
+    stream, for the pair kernel's average duration."""
     ctx.set_params(EPS, C_LIGHT, DT, mode)
     ctx.upload(x, v, m)
     tuned = ctx.autotune(2) if autotune else None     # before the timed region
     e0 = ctx.energy()
-    ctx.profile_enable(True)
     if warmup:
         ctx.step(warmup)
     ctx.sync()
-    ctx.profile_enable(True)          # drop warm-up launches from the kernel timing
     t0 = time.perf_counter()
     ctx.step(steps)
     ctx.sync()
     t1 = time.perf_counter()
+    e1 = ctx.energy()
+    ctx.profile_enable(True)
+    ctx.step(prof_steps)
+    ctx.sync()
     launches, pair_ms = ctx.profile_read()
     ctx.profile_enable(False)
-    e1 = ctx.energy()
     n = len(m)
     wall = t1 - t0
     return dict(
@@ -141,56 +157,115 @@ def load_traffic(kind, name="pair_kernel_traffic_n131073.json", shape=None):
     return d.get(kind)
 
 
-def cpu_baseline(x, v, m):
-    """The reference's own force() (compiled from its sources into
-    oracle/_ref) on the first CPU_SAMPLE_STARS stars of the workload, one
-    call on one core; the oracle restatement if that library is absent."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_bind as ob
+    return ob
 
-    n = CPU_SAMPLE_STARS + 1
-    xs, vs, ms = x[:n].copy(), v[:n].copy(), m[:n].copy()
-    # the reference has no speed of light: run it on v/c, m/c^2 (same work)
-    vr, mr = vs / C_LIGHT, ms / C_LIGHT**2
-    if ob.have_ref():
-        kind, fn = "reference", lambda: ob.ref_force(xs, vr, mr, EPS)
-    else:
-        kind, fn = "port", lambda: ob.orc_force(xs, vr, mr, EPS, 1.0)
-    t0 = time.perf_counter()
-    fn()
-    dt = time.perf_counter() - t0
+
+def _cpu_name():
     cpu = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
+    return cpu
+
+
+def cpu_force_timer(x, v, m):
+    """(kind, t(n)): the wall time of one call of the reference's own force()
+    (compiled from its sources into oracle/_ref) on the first n particles of
+    the workload, one core; the oracle restatement if that library is
+    absent.  The reference has no speed of light: it runs on v/c, m/c^2
+    (the same work)."""
+    ob = _oracle()
+    kind = "reference" if ob.have_ref() else "port"
+
+    def t(n):
+        xs, vs, ms = x[:n].copy(), v[:n].copy() / C_LIGHT, m[:n].copy() / C_LIGHT**2
+        t0 = time.perf_counter()
+        if kind == "reference":
+            ob.ref_force(xs, vs, ms, EPS)
+        else:
+            ob.orc_force(xs, vs, ms, EPS, 1.0)
+        dt = time.perf_counter() - t0
+        print(f"# cpu {kind} force() n={n}: {dt:.2f} s", file=sys.stderr, flush=True)
+        return dt
+    return kind, t
+
+
+def cpu_baseline(x, v, m, fit):
+    """One force() call on the first CPU_SAMPLE_STARS stars (the bounded
+    sample), beside `fit`: the n^2 fit over CPU_FIT_SIZES, measured
+    concurrently (cpu_fit) on another core."""
+    kind, t = cpu_force_timer(x, v, m)
+    n = CPU_SAMPLE_STARS + 1
+    dt = t(n)
     return {"value": float(n) * n / dt, "unit": "interactions/s", "cores": 1, "kind": kind,
             "sample": f"one force() call, first {CPU_SAMPLE_STARS} stars + BH of the workload "
-                      f"(n={n}, {n * n:.3e} interactions, {dt:.1f} s, gcc -O2, 1 thread, {cpu})"}
+                      f"(n={n}, {n * n:.3e} interactions, {dt:.1f} s, gcc -O2, 1 thread, "
+                      f"{_cpu_name()})",
+            "n2_fit": fit}
 
 
-def cpu_openmp(x, v, m):
+def cpu_fit(x, v, m, out):
+    """t(n) = a n^2 + b n + c through one force() call at each of
+    CPU_FIT_SIZES (one core), and its value at the workload's full n."""
+    kind, t = cpu_force_timer(x, v, m)
+    ns = np.array(CPU_FIT_SIZES, dtype=float)
+    ts = np.array([t(int(k)) for k in CPU_FIT_SIZES])
+    a, b, c = np.linalg.solve(np.stack([ns**2, ns, np.ones_like(ns)], 1), ts)
+    nf = float(N_STARS + 1)
+    tf = a * nf * nf + b * nf + c
+    out.update({"kind": kind, "cores": 1, "sizes": [int(k) for k in CPU_FIT_SIZES],
+                "seconds": [float(q) for q in ts],
+                "interactions_per_s": [float(k * k / q) for k, q in zip(ns, ts)],
+                "fit": "t(n) = a n^2 + b n + c through the three calls",
+                "a_ns_per_interaction": float(a * 1e9), "b": float(b), "c": float(c),
+                "full_n": int(nf), "full_n_seconds": float(tf),
+                "full_n_interactions_per_s": float(nf * nf / tf)})
+
+
+def oracle_rows_check(label, a, x, v, m, seed, k=40):
+    """The oracle's force rows (orc_force_rows: the FP64 CPU restatement,
+    each row O(n)) at a sample of stars: the near-BH stars, the 256-star
+    block edges and seeded random stars; max per-star relative error of `a`
+    there."""
+    ob = _oracle()
+    n = len(m)
+    d = np.linalg.norm(x[1:] - x[0], axis=1)
+    rows = np.unique(np.concatenate([[1, 2, 255, 256, 257, n - 1], 1 + np.argsort(d)[:8],
+                                     np.random.default_rng(seed).integers(1, n, k)]))
+    rows = rows[(rows >= 1) & (rows < n)].astype(np.int32)
+    ref = ob.orc_force_rows(x, v, m, EPS, C_LIGHT, rows)
+    err = np.linalg.norm(a[rows] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    print(f"# {label}: {len(rows)} oracle rows, max rel {err.max():.3e}", file=sys.stderr,
+          flush=True)
+    return float(err.max()), int(len(rows))
+
+
+def cpu_openmp(x, v, m, reserve=0):
     """Beside the one-core baseline: the oracle's force() restatement with its
     O(n^2) loops over OpenMP threads (one per core of this process's CPU
-    share, at most 16), on the first CPU_OMP_SAMPLE_STARS stars.  Not the
-    reference (which is serial); its star rows are bitwise the serial ones."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_bind as ob
-
+    share, at most 16, less `reserve` cores busy with other calls), on the
+    first CPU_OMP_SAMPLE_STARS stars.  Not the reference (which is serial);
+    its star rows are bitwise the serial ones."""
+    ob = _oracle()
     try:
         cores = len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
         cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
+    threads = max(1, min(16, cores) - reserve)
     n = CPU_OMP_SAMPLE_STARS + 1
     xs, vs, ms = x[:n].copy(), v[:n].copy(), m[:n].copy()
     t0 = time.perf_counter()
     _, ran = ob.orc_force_omp(xs, vs, ms, EPS, C_LIGHT, threads)
     dt = time.perf_counter() - t0
+    print(f"# cpu OpenMP force() n={n}: {dt:.2f} s on {ran} threads", file=sys.stderr,
+          flush=True)
     return {"value": float(n) * n / dt, "unit": "interactions/s", "cores": int(ran),
             "kind": "port",
             "sample": f"one force() call of the oracle restatement over {ran} OpenMP threads, "
@@ -489,19 +564,42 @@ def main():
     if args.drift_steps:
         print(json.dumps(run_drift(x, v, m, args.drift_steps)), flush=True)
         return
+    # the CPU baseline's calls run on their own cores beside the GPU legs
+    fit, sample, threads = {}, {}, []
+    if not args.no_cpu:
+        old_stack = threading.stack_size(256 << 20)
+        threads = [threading.Thread(target=cpu_fit, args=(x, v, m, fit), daemon=True),
+                   threading.Thread(target=lambda: sample.update(cpu_baseline(x, v, m, fit)),
+                                    daemon=True)]
+        for t in threads:
+            t.start()
+        threading.stack_size(old_stack)
     with pn.Context(capacity=n, device=0) as ctx:
         ctx.set_tuning(args.jsplit, args.tile, args.variant)
-        # accuracy of the mixed form at this state (one evaluation each)
+        # accuracy of the mixed form: at the initial state (FP32-exact
+        # positions) and after two FP64 steps (positions with all 53 bits)
+        states = {"initial": (x, v)}
         ctx.set_params(EPS, C_LIGHT, DT, pn.FP64)
         ctx.upload(x, v, m)
-        a64 = ctx.force()
-        ctx.set_params(EPS, C_LIGHT, DT, pn.MIXED)
-        ctx.upload(x, v, m)
-        a32 = ctx.force()
-        err = rel_err(a32, a64)
+        ctx.step(2)
+        states["two_steps"] = ctx.download()
+        forces = {}
+        for name, (xs, vs) in states.items():
+            for mode in (pn.FP64, pn.MIXED):
+                ctx.set_params(EPS, C_LIGHT, DT, mode)
+                ctx.upload(xs, vs, m)
+                forces[name, mode] = ctx.force()
+        err = rel_err(forces["initial", pn.MIXED], forces["initial", pn.FP64])
+        err2 = rel_err(forces["two_steps", pn.MIXED], forces["two_steps", pn.FP64])
+        print(f"# mixed vs FP64: star max {err[1:].max():.3e} (initial), "
+              f"{err2[1:].max():.3e} (two steps on)", file=sys.stderr, flush=True)
         tune = args.jsplit == 0 and args.variant == -1 and args.autotune
         fp64 = run_form(ctx, x, v, m, pn.FP64, args.steps, args.warmup, tune)
+        print(f"# FP64: {fp64['ms_per_step']:.2f} ms/step, pair {fp64['pair_avg_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
         mixed = run_form(ctx, x, v, m, pn.MIXED, args.steps, args.warmup, tune)
+        print(f"# mixed: {mixed['ms_per_step']:.2f} ms/step, pair {mixed['pair_avg_ms']:.3f} ms",
+              file=sys.stderr, flush=True)
         jsplit, tile, variant = mixed["shape"]
         shard = run_shard(ctx, x, v, m)
         for name, form in (("mixed", mixed), ("fp64", fp64)):
@@ -542,6 +640,10 @@ def main():
         "mixed_accuracy_vs_fp64": {"star_rel_err_max": float(err[1:].max()),
                                    "star_rel_err_p999": float(np.quantile(err[1:], 0.999)),
                                    "bh_rel_err": float(err[0]),
+                                   "two_steps_on": {
+                                       "star_rel_err_max": float(err2[1:].max()),
+                                       "star_rel_err_p999": float(np.quantile(err2[1:], 0.999)),
+                                       "bh_rel_err": float(err2[0])},
                                    "energy_drift_mixed": mixed["energy_drift"]},
         "pair_kernel_avg_ms": {"mixed": mixed["pair_avg_ms"], "fp64": fp64["pair_avg_ms"],
                                "launches": mixed["pair_launches"]},
@@ -553,21 +655,28 @@ def main():
                                         "fp64": 20.0 * fp64["interactions_per_s"] / 1e12},
     }
     if args.ens_steps > 0:
+        print("# ensemble leg", file=sys.stderr, flush=True)
         out["ensemble"] = run_ensemble(args.ens_steps, args.warmup, chunk=args.ens_chunk,
                                        tile=args.tile, variant=args.variant, eager=args.eager,
                                        adj_steps=args.ens_steps if args.adj_steps > 0 else 0,
                                        autotune=not args.no_ens_autotune and args.ens_chunk == 0)
     if args.adj_steps > 0:
+        print("# adjoint leg", file=sys.stderr, flush=True)
         out["adjoint"] = run_adjoint(args.adj_steps, min(16, args.adj_steps))
     if not args.no_cpu:
-        old = threading.stack_size(256 << 20)
-        res = {}
-        t = threading.Thread(target=lambda: res.setdefault("r", cpu_baseline(x, v, m)))
-        t.start()
-        t.join()
-        threading.stack_size(old)
-        out["cpu_baseline"] = res.get("r")
-        out["cpu_openmp"] = cpu_openmp(x, v, m)
+        # the checker: the oracle's rows against both forms
+        acc = {}
+        for name, (xs, vs) in states.items():
+            for mode, label in ((pn.MIXED, "mixed"), (pn.FP64, "fp64")):
+                e, k = oracle_rows_check(f"{label} {name}", forces[name, mode], xs, vs, m,
+                                         seed=len(acc))
+                acc[f"{label}_{name}"] = e
+        acc["rows_per_state"] = k
+        out["accuracy_vs_oracle_rows"] = acc
+        threads[1].join()
+        out["cpu_baseline"] = sample
+        out["cpu_openmp"] = cpu_openmp(x, v, m, reserve=1 if threads[0].is_alive() else 0)
+        threads[0].join()
     print(json.dumps(out), flush=True)
 
 

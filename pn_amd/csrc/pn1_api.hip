@@ -152,6 +152,26 @@ int kernel_variant(const pn1_ctx* c)
     return 0;
 }
 
+// Whether the mixed pair kernel combines four chunks per workgroup
+// (k_pair32q, variant bit 2, splits into a multiple of 4 chunks): a quarter
+// of the partial-sum traffic, at ~1.6% more pair-kernel time than one chunk
+// per 256-thread workgroup (k_pair32) at n = 131073 on MI355X.
+bool quad_pairs(const pn1_ctx* c, int S)
+{
+    return c->mode == PN1_MIXED && S >= QSUB && S % QSUB == 0 && (kernel_variant(c) & 4);
+}
+
+// Partial-sum slots per star of a launch split in S chunks.
+int part_slots(const pn1_ctx* c, int S) { return quad_pairs(c, S) ? S / QSUB : S; }
+
+// Workgroups of a pair launch over nib i blocks with `slots` chunks each
+// (map_block's arrangement when slots divides 8).
+int pair_grid(int slots, int nib)
+{
+    if (slots <= 8 && 8 % slots == 0) return 8 * ((nib + 8 / slots - 1) / (8 / slots));
+    return slots * nib;
+}
+
 void drop_graph(pn1_ctx* c)
 {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -190,21 +210,28 @@ void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
     // first stage of each step (the pair kernel accumulated their sums)
     float* kout = (MODE == 1 && STAGE <= 1) ? c->kr : nullptr;
     hipLaunchKernelGGL((k_fin<MODE, STAGE>), dim3(c->nblk), dim3(WG), 0, c->st, in, c->X, c->Y,
-                       c->H1, c->H2, c->m, (const void*)c->part, c->S, c->gs, c->acc, c->bhblk,
-                       kout, (const int*)c->fn, (const double*)c->xp, P, 1, c->n);
+                       c->H1, c->H2, c->m, (const void*)c->part, part_slots(c, c->S), c->gs,
+                       c->acc, c->bhblk,
+                       kout, (const int*)c->fn, (const double*)c->xp, (const int*)c->fl,
+                       (const double*)c->xh, P, 1, c->n);
     hipLaunchKernelGGL((k_bh<STAGE>), dim3(1), dim3(WG), 0, c->st, c->bhblk, c->nblk, c->gs, c->X,
                        c->Y, c->H1, c->H2, c->acc, P);
 }
 
 template <int PP, bool TS, bool EQ>
-void launch32(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S, dim3 g)
+void launch32(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S)
 {
-    hipLaunchKernelGGL((k_pair32<PP, TS, EQ>), g, dim3(WG), 0, c->st, c->j32, c->j64, c->thr, in,
-                       (float*)c->part, P, S, nib, ib0, c->tile);
+    if (quad_pairs(c, S))
+        hipLaunchKernelGGL((k_pair32q<PP, TS, EQ>), dim3(pair_grid(S / QSUB, nib)), dim3(WGQ), 0,
+                           c->st, c->j32, c->j64, c->thr, in, (float*)c->part, P, S, nib, ib0,
+                           c->tile);
+    else
+        hipLaunchKernelGGL((k_pair32<PP, TS, EQ>), dim3(pair_grid(S, nib)), dim3(WG), 0, c->st,
+                           c->j32, c->j64, c->thr, in, (float*)c->part, P, S, nib, ib0, c->tile);
 }
 
 // mixed pair kernels by [variant][cancellation sums][equal masses]
-using Launch32 = void (*)(pn1_ctx*, const double*, const Phys&, int, int, int, dim3);
+using Launch32 = void (*)(pn1_ctx*, const double*, const Phys&, int, int, int);
 const Launch32 k_launch32[16] = {
     launch32<0, false, false>, launch32<0, false, true>, launch32<0, true, false>,
     launch32<0, true, true>,   launch32<1, false, false>, launch32<1, false, true>,
@@ -219,15 +246,10 @@ const Launch32 k_launch32[16] = {
 void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S, bool ts)
 {
     const bool mixed = c->mode == PN1_MIXED;
-    int grid = S * nib;
-    if (S <= 8 && 8 % S == 0) {
-        const int per = 8 / S;
-        grid = 8 * ((nib + per - 1) / per);
-    }
-    const dim3 g(grid), blk(WG);
+    const dim3 g(pair_grid(S, nib)), blk(WG);
     const int variant = kernel_variant(c);
     if (mixed) {
-        k_launch32[(variant & 3) * 4 + (ts ? 2 : 0) + (c->eqm ? 1 : 0)](c, in, P, ib0, nib, S, g);
+        k_launch32[(variant & 3) * 4 + (ts ? 2 : 0) + (c->eqm ? 1 : 0)](c, in, P, ib0, nib, S);
     } else if (variant & 1) {
         hipLaunchKernelGGL(k_pair64<1>, g, blk, 0, c->st, c->j64, in, (double*)c->part, P, S, nib,
                            ib0);
@@ -249,6 +271,7 @@ MixPrep mix_prep(const pn1_ctx* c, const Phys& P, bool prev)
     M.mmax = c->mmax;
     M.tauf = c->tauf;
     M.lam = P.lam;
+    M.hard_rows = 1;
     return M;
 }
 
@@ -260,6 +283,7 @@ MixPrep fp64_prep()
     M.aprev = nullptr;
     M.kprev = nullptr;
     M.mmax = M.tauf = M.lam = 0.0;
+    M.hard_rows = 0;
     return M;
 }
 
@@ -275,9 +299,12 @@ void launch_prep(pn1_ctx* c, const double* in, const Phys& P)
     hipLaunchKernelGGL((k_rows_sum<NGS>), dim3(1), dim3(WG), 0, c->st, c->gblk, c->nblk, c->gs);
     if (mixed) {
         hipLaunchKernelGGL(k_flist, dim3(1), dim3(WG), 0, c->st, (const int*)c->fl,
-                           (const int*)c->fblk, c->nblk, c->n, c->flist, c->fn);
+                           (const int*)c->fblk, c->nblk, c->n, c->flist, c->hlist, c->fn);
         hipLaunchKernelGGL(k_pair_fl, dim3(c->nblk), dim3(WG), 0, c->st, (const JRec64*)c->j64,
                            in, (const int*)c->flist, (const int*)c->fn, c->xp, P);
+        hipLaunchKernelGGL(k_hard_rows, dim3(c->hgrid), dim3(WG), 0, c->st,
+                           (const JRec64*)c->j64, in, (const int*)c->hlist, (const int*)c->fn,
+                           c->xh, P);
     }
 }
 
@@ -425,7 +452,7 @@ void pn1_destroy(pn1_ctx* c)
     if (c->j32) (void)hipFree(c->j32);
     if (c->thr) (void)hipFree(c->thr);
     if (c->kr) (void)hipFree(c->kr);
-    void* fbufs[] = {c->fl, c->fblk, c->flist, c->fn, c->xp};
+    void* fbufs[] = {c->fl, c->fblk, c->flist, c->fn, c->xp, c->hlist, c->xh};
     for (void* b : fbufs)
         if (b) (void)hipFree(b);
     if (c->part) (void)hipFree(c->part);
@@ -477,8 +504,10 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
     if (!rc) rc = dalloc(c, &c->thr, cap);
     if (!rc) rc = dalloc(c, &c->kr, cap);
     if (!rc) rc = dalloc(c, &c->fl, cap);
-    if (!rc) rc = dalloc(c, &c->fblk, nb);
+    if (!rc) rc = dalloc(c, &c->fblk, 2 * nb);
     if (!rc) rc = dalloc(c, &c->flist, cap);
+    if (!rc) rc = dalloc(c, &c->hlist, cap);
+    if (!rc) rc = dalloc(c, &c->xh, (size_t)(NACC + 1) * cap);
     if (!rc) rc = dalloc(c, &c->fn, 4);
     if (!rc) rc = dalloc(c, &c->xp, (size_t)(NACC + 1) * cap);
     if (!rc && host_malloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {
@@ -512,7 +541,7 @@ int pn1_set_params(pn1_ctx* c, double eps, double cl, double dt, int mode)
 int pn1_set_tuning(pn1_ctx* c, int jsplit, int tile, int variant)
 {
     if (!c) return PN1_EINVAL;
-    if (jsplit < 0 || jsplit > 4096 || tile < 0 || tile > 65536 || variant < -1 || variant > 3)
+    if (jsplit < 0 || jsplit > 4096 || tile < 0 || tile > 65536 || variant < -1 || variant > 7)
         return seterr(c, PN1_EINVAL, "bad tuning (jsplit %d, tile %d, variant %d)", jsplit, tile,
                       variant);
     const int keep_S = c->jsplit_req, keep_tile = c->tile_req, keep_var = c->variant_req;
@@ -644,12 +673,14 @@ int pn1_force_range(pn1_ctx* c, int i0, int i1, double* a)
     const dim3 gf(nib);
     if (mixed)
         hipLaunchKernelGGL((k_fin<1, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr,
-                           (const int*)c->fn, (const double*)c->xp, P, i0, i1);
+                           c->m, (const void*)c->part, part_slots(c, S), c->gs, c->acc, c->bhblk, c->kr,
+                           (const int*)c->fn, (const double*)c->xp, (const int*)c->fl,
+                           (const double*)c->xh, P, i0, i1);
     else
         hipLaunchKernelGGL((k_fin<0, 0>), gf, dim3(WG), 0, c->st, in, c->X, c->Y, c->H1, c->H2,
-                           c->m, (const void*)c->part, S, c->gs, c->acc, c->bhblk, c->kr,
-                           (const int*)c->fn, (const double*)c->xp, P, i0, i1);
+                           c->m, (const void*)c->part, part_slots(c, S), c->gs, c->acc, c->bhblk, c->kr,
+                           (const int*)c->fn, (const double*)c->xp, (const int*)c->fl,
+                           (const double*)c->xh, P, i0, i1);
     if (!mixed) c->acc_ok = false;
     HIPCHK(c, hipGetLastError());
     const size_t cap = (size_t)c->cap, cnt = (size_t)(i1 - i0);
@@ -898,7 +929,8 @@ int pn1_force_noalloc(const double* st, const double* m, int n, double eps, doub
     hipLaunchKernelGGL((k_fin<0, 0>), dim3(nblk), dim3(WG), 0, s, st, (double*)nullptr,
                        (double*)nullptr, (double*)nullptr, (double*)nullptr, m,
                        (const void*)part, S, (const double*)gs, acc, bhblk, (float*)nullptr,
-                       (const int*)nullptr, (const double*)nullptr, P, 1, n);
+                       (const int*)nullptr, (const double*)nullptr, (const int*)nullptr,
+                       (const double*)nullptr, P, 1, n);
     hipLaunchKernelGGL((k_bh<0>), dim3(1), dim3(WG), 0, s, (const double*)bhblk, nblk,
                        (const double*)gs, (double*)nullptr, (double*)nullptr, (double*)nullptr,
                        (double*)nullptr, acc, P);

@@ -102,8 +102,13 @@ struct Phys {
 //        ~u |x| in the smooth far field, and a few u of that field.
 // A star whose previous evaluation had mscale * A_TS > PN1_HARD_K |a| (its
 // pair terms cancel to 1/30 of their magnitudes or less) takes every pair in
-// FP64 (rc2 = FLT_MAX): the FP32 rounding of the terms, ~0.1 u of their
-// magnitudes once averaged, would otherwise near 1e-6 of |a|.
+// FP64: the FP32 rounding of the terms, ~0.1 u of their magnitudes once
+// averaged, would otherwise near 1e-6 of |a|.  With MixPrep::hard_rows the
+// star's whole row is evaluated in FP64 by a pass of its own (k_hard_rows,
+// one workgroup per such star) and its FP32 lane does no extra work
+// (rc2 = dn2 = 0; k_fin takes the FP64 row instead); otherwise its FP32 lane
+// takes the FP64 branch for every pair (rc2 = FLT_MAX), which makes its whole
+// wave several times slower than the others.
 // Without a previous evaluation (after an upload) the thresholds are
 // rc2 = 0 and dn2 = FLT_MAX: no FP64 pairs, lo parts everywhere.
 #define PN1_CLOSE_KAPPA 0.25
@@ -136,6 +141,7 @@ struct MixPrep {
     const float* kprev;
     double mmax, tauf;      // the cluster's largest star mass; BH-close threshold
     double lam;
+    int hard_rows;          // stars beyond PN1_HARD_K get FP64 rows of their own
 };
 
 // ---------------------------------------------------------------------
@@ -183,6 +189,16 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* out)
 #pragma unroll
         for (int k = 0; k < N; ++k) out[k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
     }
+}
+
+// Store of a pair kernel's partial sum, read once by the finishing kernel:
+// non-temporal, so that the partials streaming out do not evict the
+// j-records from L2 (measured on MI355X at n = 131073: the FP64 pair
+// kernel's fetches fall from 1.03 GB to 0.18 GB per launch).
+template <typename T>
+__device__ __forceinline__ void stream_out(T v, T* p)
+{
+    __builtin_nontemporal_store(v, p);
 }
 
 // Workgroup -> (j chunk s, i block ib).  When S divides 8, consecutive

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
         if (g.li < w.nst) {
             double* pp = part + w.pslot * NACC + g.li;
 #pragma unroll
-            for (int k = 0; k < NACC; ++k) pp[(size_t)k * w.nst] = a[k];
+            for (int k = 0; k < NACC; ++k) stream_out(a[k], &pp[(size_t)k * w.nst]);
         }
     }
 }
@@ -124,9 +124,9 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
             float* pp = part + w.pslot * MROWS + g.li;
 #pragma unroll
             for (int k = 0; k < NACC32; ++k)
-                if (TS || k != A_TS) pp[(size_t)k * w.nst] = k <= A_NZ ? sN[k] : a[k];
+                if (TS || k != A_TS) stream_out(k <= A_NZ ? sN[k] : a[k], &pp[(size_t)k * w.nst]);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pp[(size_t)(NACC32 + k) * w.nst] = cN[k];
+            for (int k = 0; k < 3; ++k) stream_out(cN[k], &pp[(size_t)(NACC32 + k) * w.nst]);
         }
     }
 }
@@ -155,9 +155,9 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
         M.mmax = cmmax[b.c];
         M.tauf = ctauf[b.c];
     }
-    bool f = false;
+    bool f = false, hard;
     if (li < nst) {
-        f = prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, g, M);
+        f = prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, g, M, hard);
         if (M.j32) fl[off + 1 + li] = f ? 1 : 0;
     }
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ j
         if (g.li < w.nst) {
             double* pp = part + w.pslot * NACCE + g.li;
 #pragma unroll
-            for (int k = 0; k < NACCE; ++k) pp[(size_t)k * w.nst] = a[k];
+            for (int k = 0; k < NACCE; ++k) stream_out(a[k], &pp[(size_t)k * w.nst]);
         }
     }
 }

@@ -68,12 +68,19 @@ __device__ __forceinline__ void stage_update(int p, int cap, const double (&a)[3
 // known), for star masses up to mmax (see Thr).
 __device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
                                                const double* aprev, const float* kprev,
-                                               size_t cap, int p, double mmax)
+                                               size_t cap, int p, double mmax, bool rows,
+                                               bool& hard)
 {
     Thr t;
+    hard = false;
     if (!aprev) {
         t.rc2 = 0.0f;
         t.dn2 = __FLT_MAX__;
+        return t;
+    }
+    hard = !(kprev[p] <= (float)PN1_HARD_K);                   // NaN too
+    if (hard && rows) {          // the FP32 lane's sums are not used
+        t.rc2 = t.dn2 = 0.0f;
         return t;
     }
     const double ax = aprev[p], ay = aprev[cap + p], az = aprev[2 * cap + p];
@@ -82,8 +89,7 @@ __device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
     const double rc2 = mmax / (PN1_CLOSE_KAPPA * A);            // +inf if A == 0
     const double dn3 = 6.0 * 0x1p-24 * X * mmax / (PN1_LAZY_ETA * A);
     const double dn2 = cbrt(dn3 * dn3);
-    t.rc2 = (float)fmin(rc2, (double)__FLT_MAX__);
-    if (!(kprev[p] <= (float)PN1_HARD_K)) t.rc2 = __FLT_MAX__;   // NaN too
+    t.rc2 = hard ? __FLT_MAX__ : (float)fmin(rc2, (double)__FLT_MAX__);
     // close pairs are found inside the lo-part branch: dn2 >= rc2
     t.dn2 = fmaxf((float)fmin(dn2, (double)__FLT_MAX__), t.rc2);
     return t;
@@ -95,14 +101,16 @@ __device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
 // acceleration M.aprev (SoA [3][cap]) and cancellation ratio M.kprev, and
 // its mixed record in units of the cluster's largest star mass M.mmax;
 // returns whether the star is close enough to the BH to be taken out of the
-// FP32 pass (see PN1_FLAG_ETA), in which case its mixed record is neutral.
+// FP32 pass (see PN1_FLAG_ETA), in which case its mixed record is neutral;
+// hard: whether its pair terms cancel beyond PN1_HARD_K.
 __device__ __forceinline__ bool prep_star(const double* __restrict__ st,
                                           const double* __restrict__ m, int cap, int p,
                                           int b0, double m0, double eps2,
                                           JRec64* __restrict__ j64, double (&g)[NGS],
-                                          const MixPrep& M)
+                                          const MixPrep& M, bool& hard)
 {
 #pragma clang fp contract(off)
+    hard = false;
     const double x = st[p], y = st[cap + p], z = st[2 * cap + p];
     const double vx = st[3 * cap + p], vy = st[4 * cap + p], vz = st[5 * cap + p];
     const double mp = m[p];
@@ -148,12 +156,14 @@ __device__ __forceinline__ bool prep_star(const double* __restrict__ st,
         h.Qdx = (float)r.Qdx; h.Qdy = (float)r.Qdy; h.Qdz = (float)r.Qdz;
     }
     M.j32[p] = h;
-    M.thr[p] = pair_thresholds(x, y, z, M.aprev, M.kprev, (size_t)cap, p, M.mmax);
+    M.thr[p] = pair_thresholds(x, y, z, M.aprev, M.kprev, (size_t)cap, p, M.mmax,
+                               M.hard_rows != 0, hard);
     return flagged;
 }
 
-// fblk (mixed form): per block, the number of stars taken out of the FP32
-// pass; fl: per star, whether it is one of them.  The BH mass is m[0].
+// Mixed form: fl per star, bit 0 whether it is taken out of the FP32 pass
+// (BH-close), bit 1 whether it gets an FP64 row of its own (hard); fblk per
+// block the number of each, [2 b] and [2 b + 1].  The BH mass is m[0].
 __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
                                             const double* __restrict__ m, Phys P,
                                             JRec64* __restrict__ j64,
@@ -164,15 +174,20 @@ __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
     double g[NGS];
 #pragma unroll
     for (int k = 0; k < NGS; ++k) g[k] = 0.0;
-    bool f = false;
+    bool f = false, hard = false;
     if (p < P.n) {
-        f = prep_star(st, m, P.cap, p, 0, m[0], P.eps2, j64, g, M);
-        if (M.j32) fl[p] = f ? 1 : 0;
+        f = prep_star(st, m, P.cap, p, 0, m[0], P.eps2, j64, g, M, hard);
+        hard = hard && M.hard_rows;
+        if (M.j32) fl[p] = (f ? 1 : 0) | (hard ? 2 : 0);
     }
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
     if (M.j32) {
         const int cnt = __syncthreads_count(f);
-        if (threadIdx.x == 0) fblk[blockIdx.x] = cnt;
+        const int hcnt = __syncthreads_count(hard);
+        if (threadIdx.x == 0) {
+            fblk[2 * blockIdx.x] = cnt;
+            fblk[2 * blockIdx.x + 1] = hcnt;
+        }
     }
 }
 
@@ -198,9 +213,11 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total)
     return base + incl - v;
 }
 
-// The flagged stars of blocks [b0, b1) (block b covers the stars first + 0
-// .. first + WG - 1 of star index p0 + b * WG) listed in increasing order
-// at out[0 ..), their number in *nout.  One workgroup.
+// The stars with bit BIT of fl set in blocks [b0, b1) (block b covers the
+// stars p0 + (b - b0) * WG .. + WG - 1, its count of them at
+// fblk[b * STRIDE + BIT]) listed in increasing order at out[0 ..), their
+// number in *nout.  One workgroup.
+template <int BIT = 0, int STRIDE = 1>
 __device__ __forceinline__ void list_flagged(const int* __restrict__ fl,
                                              const int* __restrict__ fblk, int b0, int b1,
                                              int p0, int pend, int* __restrict__ out,
@@ -209,25 +226,29 @@ __device__ __forceinline__ void list_flagged(const int* __restrict__ fl,
     int base = 0;
     for (int bb = b0; bb < b1; bb += WG) {
         const int b = bb + (int)threadIdx.x;
-        const int cnt = b < b1 ? fblk[b] : 0;
+        const int cnt = b < b1 ? fblk[b * STRIDE + BIT] : 0;
         int total;
         const int at = base + block_excl_scan(cnt, &total);
         if (cnt > 0) {
             int k = at;
             const int q0 = p0 + (b - b0) * WG;
             for (int q = q0; q < q0 + WG && q < pend; ++q)
-                if (fl[q]) out[k++] = q;
+                if ((fl[q] >> BIT) & 1) out[k++] = q;
         }
         base += total;
     }
     if (threadIdx.x == 0) *nout = base;
 }
 
+// The BH-close stars (flist, fn[0]) and the hard stars (hlist, fn[1]) of
+// k_prep's flags.
 __global__ __launch_bounds__(WG) void k_flist(const int* __restrict__ fl,
                                              const int* __restrict__ fblk, int nblk, int n,
-                                             int* __restrict__ flist, int* __restrict__ fn)
+                                             int* __restrict__ flist, int* __restrict__ hlist,
+                                             int* __restrict__ fn)
 {
-    list_flagged(fl, fblk, 0, nblk, 1, n, flist, fn);
+    list_flagged<0, 2>(fl, fblk, 0, nblk, 1, n, flist, fn);
+    list_flagged<1, 2>(fl, fblk, 0, nblk, 1, n, hlist, fn + 1);
 }
 
 // Fixed-order sum of nblk rows of N doubles (one workgroup).
@@ -368,7 +389,7 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
     }
     if (li < n) {
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) part[(size_t)(s * NACC + k) * P.cap + li] = a[k];
+        for (int k = 0; k < NACC; ++k) stream_out(a[k], &part[(size_t)(s * NACC + k) * P.cap + li]);
     }
 }
 
@@ -663,9 +684,96 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
         float* ps = part + (size_t)s * MROWS * cap + li;
 #pragma unroll
         for (int k = 0; k < NACC32; ++k)
-            if (TS || k != A_TS) ps[k * cap] = k <= A_NZ ? sumN[k] : a[k];
+            if (TS || k != A_TS) stream_out(k <= A_NZ ? sumN[k] : a[k], &ps[k * cap]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) ps[(NACC32 + k) * cap] = cmpN[k];
+        for (int k = 0; k < 3; ++k) stream_out(cmpN[k], &ps[(NACC32 + k) * cap]);
+    }
+}
+
+// The same pairs in workgroups of four i-waves x four consecutive j
+// chunks (1024 threads): wave w takes the i-stars of i-wave w % 4 of block
+// ib against chunk 4 s4 + w / 4 of the split in S chunks, so every wave
+// does what it does in k_pair32 (and its FP32 sums are the same bits), but
+// the four chunk sums of each star are combined in LDS and written as one
+// partial of the split in S / 4 chunks: a quarter of k_pair32's partial-sum
+// traffic.  The combination is in FP64: the Newtonian totals s - c are
+// written back as a hi/lo float pair (s = hi, c = -lo, so that k_fin's
+// s - c is the FP64 total to ~2^-48), the other sums rounded once to FP32.
+constexpr int QSUB = 4;
+constexpr int WGQ = QSUB * WG;
+template <int PP, bool TS, bool EQ>
+__global__ __launch_bounds__(WGQ) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_pair32q(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64,
+               const Thr* __restrict__ thr, const double* __restrict__ st,
+               float* __restrict__ part, Phys P, int S, int nIblk, int ib0, int tile)
+{
+    __shared__ float red[QSUB - 1][MROWS][WG];
+    const int S4 = S / QSUB;
+    int s4, ib;
+    map_block(blockIdx.x, S4, s4, ib);
+    if (ib >= nIblk) return;                  // the whole workgroup
+    ib += ib0;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int iw = wave & 3, sub = wave >> 2, lane = threadIdx.x & 63;
+    const int n = P.n;
+    const int li = ib * WG + iw * 64 + lane + 1;
+    const int i = li < n ? li : n - 1;
+    const I32 I = load_i32(j64, thr, st, P.cap, i, P.eps2);
+    JRange R;
+    {
+        const int nst = n - 1, clen = (nst + S - 1) / S, sc = s4 * QSUB + sub;
+        R.j0 = 1 + sc * clen;
+        R.j1 = min(n, R.j0 + clen);
+        const int wb = ib * WG + iw * 64 + 1;
+        R.w0 = max(R.j0, wb);
+        R.w1 = min(R.j1, wb + 64);
+    }
+    const float eps2 = P.eps2f;
+    const double eps2d = P.eps2, rscale = 1.0 / P.mmax;
+    float a[NACC32], sumN[3], cmpN[3];
+#pragma unroll
+    for (int k = 0; k < NACC32; ++k) a[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sumN[k] = cmpN[k] = 0.0f;
+    int cnt = 0;
+    if (R.w0 < R.w1) {
+        run32<false, PP, TS, EQ>(jr, j64, R.j0, R.w0, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
+        run32<true, PP, TS, EQ>(jr, j64, R.w0, R.w1, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
+        run32<false, PP, TS, EQ>(jr, j64, R.w1, R.j1, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
+    } else if (R.j0 < R.j1) {
+        run32<false, PP, TS, EQ>(jr, j64, R.j0, R.j1, i, I, a, eps2, eps2d, rscale, sumN, cmpN, tile, cnt);
+    }
+    fold_newton(a, sumN, cmpN);
+    const int col = iw * 64 + lane;
+    if (sub > 0) {
+#pragma unroll
+        for (int k = 0; k < NACC32; ++k)
+            if (TS || k != A_TS) red[sub - 1][k][col] = k <= A_NZ ? sumN[k] : a[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) red[sub - 1][NACC32 + k][col] = cmpN[k];
+    }
+    __syncthreads();
+    if (sub == 0 && li < n) {
+        const size_t cap = (size_t)P.cap;
+        float* ps = part + (size_t)s4 * MROWS * cap + li;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double t = (double)sumN[k] - (double)cmpN[k];
+#pragma unroll
+            for (int q = 0; q < QSUB - 1; ++q)
+                t += (double)red[q][k][col] - (double)red[q][NACC32 + k][col];
+            const float hi = (float)t;
+            stream_out(hi, &ps[k * cap]);
+            stream_out(-(float)(t - (double)hi), &ps[(NACC32 + k) * cap]);
+        }
+#pragma unroll
+        for (int k = A_NZ + 1; k < NACC32; ++k) {
+            if (!TS && k == A_TS) continue;
+            double t = (double)a[k];
+#pragma unroll
+            for (int q = 0; q < QSUB - 1; ++q) t += (double)red[q][k][col];
+            stream_out((float)t, &ps[k * cap]);
+        }
     }
 }
 
@@ -711,6 +819,42 @@ __global__ __launch_bounds__(WG) void k_pair_fl(const JRec64* __restrict__ jr,
     flagged_pairs(jr, st, P.cap, i, P.eps2, flist, nf, xp, (size_t)P.cap, i);
 }
 
+// FP64 rows of the hard stars hlist[0 .. fn[1]) (see PN1_HARD_K): every
+// pair of each with the FP64 form's body (BH-close stars included, with
+// their full records), one workgroup per star at a time, the lanes striding
+// over j; per star the NACC sums and sum m_j w^2 into xh [NACC + 1][cap].
+__global__ __launch_bounds__(WG) void k_hard_rows(const JRec64* __restrict__ jr,
+                                                 const double* __restrict__ st,
+                                                 const int* __restrict__ hlist,
+                                                 const int* __restrict__ fn,
+                                                 double* __restrict__ xh, Phys P)
+{
+    const int nh = fn[1];
+    __shared__ double row[NACC + 1];
+    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
+        const int i = hlist[h];
+        const I64 I = load_i64(jr, st, P.cap, i, P.eps2);
+        double a[NACC], ts = 0.0;
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+        for (int j = 1 + (int)threadIdx.x; j < P.n; j += WG) {
+            body64<true>(jr[j], j, i, I, a, P.eps2);
+            if (j != i) {
+                const double rx = jr[j].x - I.x, ry = jr[j].y - I.y, rz = jr[j].z - I.z;
+                ts += jr[j].m / (rx * rx + ry * ry + rz * rz + P.eps2);
+            }
+        }
+        double v[NACC + 1];
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) v[k] = a[k];
+        v[NACC] = ts;
+        block_sum<NACC + 1>(v, row);
+        __syncthreads();
+        if (threadIdx.x <= NACC) xh[(size_t)threadIdx.x * P.cap + i] = row[threadIdx.x];
+        __syncthreads();
+    }
+}
+
 // Sum of the S chunk partials of one star.  `part` points at the
 // cluster's partial area, `stride` is its per-sum stride and `idx` the
 // star's index in it.  FP64: [S][NACC][stride]; mixed: [S][MROWS][stride]
@@ -744,6 +888,15 @@ __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S,
             if (k <= A_UZ || k == A_KS || k == A_PHI || k == A_W3) A[k] *= mscale;
         ts *= mscale;
     }
+}
+
+// A hard star's FP64 row (k_hard_rows).
+__device__ __forceinline__ void load_row(const double* __restrict__ xh, size_t stride, int idx,
+                                         double (&A)[NACC], double& ts)
+{
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) A[k] = xh[(size_t)k * stride + idx];
+    ts = xh[(size_t)NACC * stride + idx];
 }
 
 // Adds one star's FP64 sums of its pairs with the stars taken out of the
@@ -867,7 +1020,8 @@ __device__ __forceinline__ FinC fin_consts(const Phys& P, double m0)
 // kout (mixed form, evaluations that accumulated A_TS; else null): each
 // star's cancellation ratio, for the next evaluations' pair thresholds.
 // Mixed form: plus the FP64 sums xp of the pairs with the stars taken out of
-// the FP32 pass (when *fn > 0).
+// the FP32 pass (when fn[0] > 0); a hard star (fl bit 1) takes its FP64 row
+// xh instead of both.
 template <int MODE, int STAGE>
 __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double* Y, double* H1,
                                            double* H2, const double* __restrict__ m,
@@ -876,15 +1030,21 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
                                            double* __restrict__ acc,
                                            double* __restrict__ bhblk, float* __restrict__ kout,
                                            const int* __restrict__ fn,
-                                           const double* __restrict__ xp, Phys P, int p0,
+                                           const double* __restrict__ xp,
+                                           const int* __restrict__ fl,
+                                           const double* __restrict__ xh, Phys P, int p0,
                                            int p1)
 {
     const int p = p0 + blockIdx.x * WG + threadIdx.x;
     double cb[3] = {0.0, 0.0, 0.0};
     if (p < p1) {
         double S_[NACC], ts, a[3];
-        gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
-        if (MODE == 1 && *fn > 0) add_flagged(xp, (size_t)P.cap, p, S_, ts);
+        if (MODE == 1 && (fl[p] & 2)) {
+            load_row(xh, (size_t)P.cap, p, S_, ts);
+        } else {
+            gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
+            if (MODE == 1 && fn[0] > 0) add_flagged(xp, (size_t)P.cap, p, S_, ts);
+        }
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, m[0]), acc, cb, a);
         if (MODE == 1 && kout) kout[p] = cancel_ratio(ts, a);
     }
@@ -998,7 +1158,7 @@ __global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
     }
     if (li < n) {
 #pragma unroll
-        for (int k = 0; k < NACCE; ++k) part[(size_t)(s * NACCE + k) * cap + li] = a[k];
+        for (int k = 0; k < NACCE; ++k) stream_out(a[k], &part[(size_t)(s * NACCE + k) * cap + li]);
     }
 }
 

@@ -154,3 +154,36 @@ def test_unequal_masses():
     x, v = stepped(x, v, m, eps, c, dt, 2)
     a64, a32 = forces(x, v, m, eps, c)
     check("unequal masses", x, v, m, eps, c, a64, a32, 7)
+
+
+@pytest.mark.parametrize("workload", ["C2", "C3"])
+def test_four_chunk_workgroups(workload):
+    """Variant bit 2 (four j chunks per 1024-thread workgroup, their sums
+    combined in LDS in FP64): every wave computes the same FP32 sums as with
+    one chunk per workgroup, so the result differs from the default kernel
+    only by the rounding of the combined 1PN sums to FP32 (<= 1e-8 here),
+    and keeps the bound against FP64 and the oracle rows."""
+    if workload == "C2":
+        eps, c, dt = 1e-3, 100.0, 2.0**-10
+        x, v, m = pn.ic_uniform_sphere(10240, 2, R=1.0, Q=0.5, m_bh=0.01, eps=eps)
+    else:
+        eps, c, dt = 1e-4, 50.0, 2.0**-12
+        x, v, m = workloads.plummer_cluster()
+    x, v = stepped(x, v, m, eps, c, dt, 2)
+    print(f"case: {workload} +2, four-chunk workgroups", flush=True)
+    with pn.Context(len(m)) as ctx:
+        ctx.set_params(eps, c, dt, pn.FP64)
+        ctx.upload(x, v, m)
+        a64 = ctx.force()
+        out = {}
+        for var in (0, 4):
+            ctx.set_params(eps, c, dt, pn.MIXED)
+            ctx.set_tuning(64, 0, var)
+            ctx.upload(x, v, m)
+            out[var] = ctx.force()
+            assert ctx.get_tuning()[2] == var
+        ctx.set_tuning(0, 0, -1)
+    d = rel_err(out[4], out[0])
+    print(f"{workload}: four-chunk vs one-chunk workgroups max rel {d.max():.3e}", flush=True)
+    assert d.max() <= 1e-8
+    check(f"{workload} +2 four-chunk", x, v, m, eps, c, a64, out[4], 9)
