@@ -49,12 +49,12 @@ __device__ __forceinline__ EGeom ens_geom(const EItem& w)
 }
 
 template <int PP>
-__global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr,
-                                                  const double* __restrict__ st,
-                                                  double* __restrict__ part,
-                                                  const EItem* __restrict__ items,
-                                                  unsigned int* __restrict__ counter,
-                                                  const int* __restrict__ qoff, Phys P)
+__device__ __forceinline__ void ens_pair64(const JRec64* __restrict__ jr,
+                                           const double* __restrict__ st,
+                                           double* __restrict__ part,
+                                           const EItem* __restrict__ items,
+                                           unsigned int* __restrict__ counter,
+                                           const int* __restrict__ qoff, Phys P)
 {
     int q = 0;
     for (;;) {
@@ -85,16 +85,15 @@ __global__ __launch_bounds__(WG) void k_ens_pair64(const JRec64* __restrict__ jr
 }
 
 template <int PP, bool TS, bool EQ>
-__global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr,
-                                                  const JRec64* __restrict__ j64,
-                                                  const Thr* __restrict__ thr,
-                                                  const double* __restrict__ st,
-                                                  float* __restrict__ part,
-                                                  const EItem* __restrict__ items,
-                                                  unsigned int* __restrict__ counter,
-                                                  const int* __restrict__ qoff,
-                                                  const double* __restrict__ cmmax, Phys P,
-                                                  int tile)
+__device__ __forceinline__ void ens_pair32(const JRec32* __restrict__ jr,
+                                           const JRec64* __restrict__ j64,
+                                           const Thr* __restrict__ thr,
+                                           const double* __restrict__ st,
+                                           float* __restrict__ part,
+                                           const EItem* __restrict__ items,
+                                           unsigned int* __restrict__ counter,
+                                           const int* __restrict__ qoff,
+                                           const double* __restrict__ cmmax, Phys P, int tile)
 {
     int q = 0;
     for (;;) {
@@ -129,6 +128,53 @@ __global__ __launch_bounds__(WG) void k_ens_pair32(const JRec32* __restrict__ jr
             for (int k = 0; k < 3; ++k) stream_out(cN[k], &pp[(size_t)(NACC32 + k) * w.nst]);
         }
     }
+}
+
+// The kernels.  The persistent loop keeps an item's geometry live beside
+// the pair sums, which left to itself the compiler pays for with registers
+// (66 to 99 VGPRs, 4 to 7 waves per SIMD against the single-cluster
+// kernels' 60 and 72).  The forms with one j-record at a time are held to
+// the single-cluster kernels' occupancy: 8 waves per SIMD for the mixed
+// kernel (64 VGPRs, no spills), 7 for the FP64 one; the two-in-flight forms
+// (variant bit 0) are left to the compiler, which would spill them there.
+template <int PP>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(7, 8)))
+void k_ens_pair64(const JRec64* __restrict__ jr, const double* __restrict__ st,
+                  double* __restrict__ part, const EItem* __restrict__ items,
+                  unsigned int* __restrict__ counter, const int* __restrict__ qoff, Phys P)
+{
+    ens_pair64<PP>(jr, st, part, items, counter, qoff, P);
+}
+
+template <int PP>
+__global__ __launch_bounds__(WG)
+void k_ens_pair64x(const JRec64* __restrict__ jr, const double* __restrict__ st,
+                   double* __restrict__ part, const EItem* __restrict__ items,
+                   unsigned int* __restrict__ counter, const int* __restrict__ qoff, Phys P)
+{
+    ens_pair64<PP>(jr, st, part, items, counter, qoff, P);
+}
+
+template <int PP, bool TS, bool EQ>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_ens_pair32(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64,
+                  const Thr* __restrict__ thr, const double* __restrict__ st,
+                  float* __restrict__ part, const EItem* __restrict__ items,
+                  unsigned int* __restrict__ counter, const int* __restrict__ qoff,
+                  const double* __restrict__ cmmax, Phys P, int tile)
+{
+    ens_pair32<PP, TS, EQ>(jr, j64, thr, st, part, items, counter, qoff, cmmax, P, tile);
+}
+
+template <int PP, bool TS, bool EQ>
+__global__ __launch_bounds__(WG)
+void k_ens_pair32x(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64,
+                   const Thr* __restrict__ thr, const double* __restrict__ st,
+                   float* __restrict__ part, const EItem* __restrict__ items,
+                   unsigned int* __restrict__ counter, const int* __restrict__ qoff,
+                   const double* __restrict__ cmmax, Phys P, int tile)
+{
+    ens_pair32<PP, TS, EQ>(jr, j64, thr, st, part, items, counter, qoff, cmmax, P, tile);
 }
 
 // M: the mixed form's inputs (M.j32 null: FP64), with each cluster's largest
