@@ -45,6 +45,9 @@ struct pn1_ens {
     // FP64 sums of their pairs [NACC + 1][cap]
     int *fl = nullptr, *fblk = nullptr, *flist = nullptr, *fn = nullptr;
     double* xp = nullptr;
+    // hard stars (PN1_HARD_K): per-cluster lists and FP64 rows [NACC + 1][cap]
+    int* hlist = nullptr;
+    double* xh = nullptr;
     int fblk_cap = 0, gblk_cap = 0, bhblk_cap = 0, eblk_cap = 0;
     pn1dev::JRec64* j64 = nullptr;
     pn1dev::JRec32* j32 = nullptr;

@@ -201,29 +201,79 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
         M.mmax = cmmax[b.c];
         M.tauf = ctauf[b.c];
     }
-    bool f = false, hard;
+    bool f = false, hard = false;
     if (li < nst) {
         f = prep_star(st, m, P.cap, off + 1 + li, off, cm0[b.c], P.eps2, j64, g, M, hard);
-        if (M.j32) fl[off + 1 + li] = f ? 1 : 0;
+        hard = hard && M.hard_rows;
+        if (M.j32) fl[off + 1 + li] = (f ? 1 : 0) | (hard ? 2 : 0);
     }
     block_sum<NGS>(g, gblk + (size_t)blockIdx.x * NGS);
     if (M.j32) {
         const int cnt = __syncthreads_count(f);
-        if (threadIdx.x == 0) fblk[blockIdx.x] = cnt;
+        const int hcnt = __syncthreads_count(hard);
+        if (threadIdx.x == 0) {
+            fblk[2 * blockIdx.x] = cnt;
+            fblk[2 * blockIdx.x + 1] = hcnt;
+        }
     }
 }
 
-// One workgroup per cluster: its flagged stars, in increasing order, at
-// flist[coff[c] ..), their number in fn[c].
+// One workgroup per cluster: its BH-close stars, in increasing order, at
+// flist[coff[c] ..), their number in fn[c]; its hard stars likewise at
+// hlist[coff[c] ..), their number in hn[c].
 __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
                                                  const int* __restrict__ fblk,
                                                  const int* __restrict__ cblk,
                                                  const int* __restrict__ coff,
-                                                 int* __restrict__ flist, int* __restrict__ fn)
+                                                 int* __restrict__ flist, int* __restrict__ fn,
+                                                 int* __restrict__ hlist, int* __restrict__ hn)
 {
     const int c = blockIdx.x;
-    list_flagged(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1, coff[c + 1], flist + coff[c],
-                 fn + c);
+    list_flagged<0, 2>(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1, coff[c + 1],
+                       flist + coff[c], fn + c);
+    list_flagged<1, 2>(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1, coff[c + 1],
+                       hlist + coff[c], hn + c);
+}
+
+// FP64 rows of every cluster's hard stars (as k_hard_rows, j over the
+// star's own cluster): the workgroups take the concatenated lists in turn.
+__global__ __launch_bounds__(WG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
+                                                     const double* __restrict__ st,
+                                                     const int* __restrict__ coff, int nc,
+                                                     const int* __restrict__ hlist,
+                                                     const int* __restrict__ hn,
+                                                     double* __restrict__ xh, Phys P)
+{
+    __shared__ double row[NACC + 1];
+    const int G = (int)gridDim.x;
+    int base = 0;
+    for (int c = 0; c < nc; ++c) {
+        const int nh = hn[c];
+        const int off = coff[c], end = coff[c + 1];
+        for (int h = (((int)blockIdx.x - base) % G + G) % G; h < nh; h += G) {
+            const int i = hlist[off + h];
+            const I64 I = load_i64(jr, st, P.cap, i, P.eps2);
+            double a[NACC], ts = 0.0;
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+            for (int j = off + 1 + (int)threadIdx.x; j < end; j += WG) {
+                body64<true>(jr[j], j, i, I, a, P.eps2);
+                if (j != i) {
+                    const double rx = jr[j].x - I.x, ry = jr[j].y - I.y, rz = jr[j].z - I.z;
+                    ts += jr[j].m / (rx * rx + ry * ry + rz * rz + P.eps2);
+                }
+            }
+            double v[NACC + 1];
+#pragma unroll
+            for (int k = 0; k < NACC; ++k) v[k] = a[k];
+            v[NACC] = ts;
+            block_sum<NACC + 1>(v, row);
+            __syncthreads();
+            if (threadIdx.x <= NACC) xh[(size_t)threadIdx.x * P.cap + i] = row[threadIdx.x];
+            __syncthreads();
+        }
+        base += nh;
+    }
 }
 
 // Per star block: its stars' FP64 pairs with their cluster's flagged stars
@@ -285,7 +335,9 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
                                                const double* __restrict__ cmmax,
                                                float* __restrict__ kout,
                                                const int* __restrict__ fn,
-                                               const double* __restrict__ xp, Phys P,
+                                               const double* __restrict__ xp,
+                                               const int* __restrict__ fl,
+                                               const double* __restrict__ xh, Phys P,
                                                EConst E)
 {
     const EBlk b = blk[blockIdx.x];
@@ -294,11 +346,16 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
     double cb[3] = {0.0, 0.0, 0.0};
     if (li < nst) {
         double S_[NACC], ts, a[3];
-        const void* base = MODE == 0 ? (const void*)((const double*)part + poff[b.c] * NACC)
-                                     : (const void*)((const float*)part + poff[b.c] * MROWS);
-        gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li,
-                         MODE == 1 ? cmmax[b.c] : 1.0, S_, ts);
-        if (MODE == 1 && fn[b.c] > 0) add_flagged(xp, (size_t)P.cap, off + 1 + li, S_, ts);
+        if (MODE == 1 && (fl[off + 1 + li] & 2)) {
+            load_row(xh, (size_t)P.cap, off + 1 + li, S_, ts);
+        } else {
+            const void* base = MODE == 0
+                                   ? (const void*)((const double*)part + poff[b.c] * NACC)
+                                   : (const void*)((const float*)part + poff[b.c] * MROWS);
+            gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li,
+                             MODE == 1 ? cmmax[b.c] : 1.0, S_, ts);
+            if (MODE == 1 && fn[b.c] > 0) add_flagged(xp, (size_t)P.cap, off + 1 + li, S_, ts);
+        }
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, off + 1 + li, off, S_,
                         gs + (size_t)b.c * NGS, fin_consts(P, cm0[b.c]), acc, cb, a);
         if (MODE == 1 && kout) kout[off + 1 + li] = cancel_ratio(ts, a);
