@@ -302,7 +302,7 @@ void launch_prep(pn1_ctx* c, const double* in, const Phys& P)
                            (const int*)c->fblk, c->nblk, c->n, c->flist, c->hlist, c->fn);
         hipLaunchKernelGGL(k_pair_fl, dim3(c->nblk), dim3(WG), 0, c->st, (const JRec64*)c->j64,
                            in, (const int*)c->flist, (const int*)c->fn, c->xp, P);
-        hipLaunchKernelGGL(k_hard_rows, dim3(c->hgrid), dim3(WG), 0, c->st,
+        hipLaunchKernelGGL(k_hard_rows, dim3(c->hgrid), dim3(HWG), 0, c->st,
                            (const JRec64*)c->j64, in, (const int*)c->hlist, (const int*)c->fn,
                            c->xh, P);
     }

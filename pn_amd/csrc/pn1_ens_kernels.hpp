@@ -237,41 +237,20 @@ __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
 
 // FP64 rows of every cluster's hard stars (as k_hard_rows, j over the
 // star's own cluster): the workgroups take the concatenated lists in turn.
-__global__ __launch_bounds__(WG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
-                                                     const double* __restrict__ st,
-                                                     const int* __restrict__ coff, int nc,
-                                                     const int* __restrict__ hlist,
-                                                     const int* __restrict__ hn,
-                                                     double* __restrict__ xh, Phys P)
+__global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
+                                                      const double* __restrict__ st,
+                                                      const int* __restrict__ coff, int nc,
+                                                      const int* __restrict__ hlist,
+                                                      const int* __restrict__ hn,
+                                                      double* __restrict__ xh, Phys P)
 {
-    __shared__ double row[NACC + 1];
     const int G = (int)gridDim.x;
     int base = 0;
     for (int c = 0; c < nc; ++c) {
         const int nh = hn[c];
         const int off = coff[c], end = coff[c + 1];
-        for (int h = (((int)blockIdx.x - base) % G + G) % G; h < nh; h += G) {
-            const int i = hlist[off + h];
-            const I64 I = load_i64(jr, st, P.cap, i, P.eps2);
-            double a[NACC], ts = 0.0;
-#pragma unroll
-            for (int k = 0; k < NACC; ++k) a[k] = 0.0;
-            for (int j = off + 1 + (int)threadIdx.x; j < end; j += WG) {
-                body64<true>(jr[j], j, i, I, a, P.eps2);
-                if (j != i) {
-                    const double rx = jr[j].x - I.x, ry = jr[j].y - I.y, rz = jr[j].z - I.z;
-                    ts += jr[j].m / (rx * rx + ry * ry + rz * rz + P.eps2);
-                }
-            }
-            double v[NACC + 1];
-#pragma unroll
-            for (int k = 0; k < NACC; ++k) v[k] = a[k];
-            v[NACC] = ts;
-            block_sum<NACC + 1>(v, row);
-            __syncthreads();
-            if (threadIdx.x <= NACC) xh[(size_t)threadIdx.x * P.cap + i] = row[threadIdx.x];
-            __syncthreads();
-        }
+        for (int h = (((int)blockIdx.x - base) % G + G) % G; h < nh; h += G)
+            hard_row(jr, st, P.cap, P.eps2, hlist[off + h], off + 1, end, xh);
         base += nh;
     }
 }

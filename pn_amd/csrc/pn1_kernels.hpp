@@ -819,40 +819,68 @@ __global__ __launch_bounds__(WG) void k_pair_fl(const JRec64* __restrict__ jr,
     flagged_pairs(jr, st, P.cap, i, P.eps2, flist, nf, xp, (size_t)P.cap, i);
 }
 
-// FP64 rows of the hard stars hlist[0 .. fn[1]) (see PN1_HARD_K): every
-// pair of each with the FP64 form's body (BH-close stars included, with
-// their full records), one workgroup per star at a time, the lanes striding
-// over j; per star the NACC sums and sum m_j w^2 into xh [NACC + 1][cap].
-__global__ __launch_bounds__(WG) void k_hard_rows(const JRec64* __restrict__ jr,
-                                                 const double* __restrict__ st,
-                                                 const int* __restrict__ hlist,
-                                                 const int* __restrict__ fn,
-                                                 double* __restrict__ xh, Phys P)
+// FP64 row of hard star i (see PN1_HARD_K) over the records [j0, j1) with
+// the FP64 form's body (BH-close stars included, with their full records):
+// the NACC sums and sum m_j w^2 into xh [NACC + 1][cap] at i.  One
+// workgroup of HWG threads per row, two records per lane in flight (the
+// loads of a row are not shared by other lanes, so their latency is hidden
+// only by having many in flight).
+constexpr int HWG = 1024;
+__device__ __forceinline__ void hard_row(const JRec64* __restrict__ jr,
+                                         const double* __restrict__ st, int cap, double eps2,
+                                         int i, int j0, int j1, double* __restrict__ xh)
+{
+    __shared__ double wsum[NACC + 1][HWG / 64];
+    const I64 I = load_i64(jr, st, cap, i, eps2);
+    double a[NACC], ts = 0.0;
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) a[k] = 0.0;
+    auto pair = [&](const JRec64& J, int j) {
+        body64<true>(J, j, i, I, a, eps2);
+        if (j != i) {
+            const double rx = J.x - I.x, ry = J.y - I.y, rz = J.z - I.z;
+            ts += J.m / (rx * rx + ry * ry + rz * rz + eps2);
+        }
+    };
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += 2 * HWG) {
+        const int jb = j + HWG;
+        const JRec64 A = jr[j];
+        const JRec64 B = jr[jb < j1 ? jb : j];
+        pair(A, j);
+        if (jb < j1) pair(B, jb);
+    }
+    double v[NACC + 1];
+#pragma unroll
+    for (int k = 0; k < NACC; ++k) v[k] = a[k];
+    v[NACC] = ts;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k <= NACC; ++k) {
+        double t = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        if (lane == 0) wsum[k][wid] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x <= NACC) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < HWG / 64; ++w) t += wsum[threadIdx.x][w];
+        xh[(size_t)threadIdx.x * cap + i] = t;
+    }
+    __syncthreads();
+}
+
+// The hard stars hlist[0 .. fn[1]), one workgroup per star at a time.
+__global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr,
+                                                  const double* __restrict__ st,
+                                                  const int* __restrict__ hlist,
+                                                  const int* __restrict__ fn,
+                                                  double* __restrict__ xh, Phys P)
 {
     const int nh = fn[1];
-    __shared__ double row[NACC + 1];
-    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
-        const int i = hlist[h];
-        const I64 I = load_i64(jr, st, P.cap, i, P.eps2);
-        double a[NACC], ts = 0.0;
-#pragma unroll
-        for (int k = 0; k < NACC; ++k) a[k] = 0.0;
-        for (int j = 1 + (int)threadIdx.x; j < P.n; j += WG) {
-            body64<true>(jr[j], j, i, I, a, P.eps2);
-            if (j != i) {
-                const double rx = jr[j].x - I.x, ry = jr[j].y - I.y, rz = jr[j].z - I.z;
-                ts += jr[j].m / (rx * rx + ry * ry + rz * rz + P.eps2);
-            }
-        }
-        double v[NACC + 1];
-#pragma unroll
-        for (int k = 0; k < NACC; ++k) v[k] = a[k];
-        v[NACC] = ts;
-        block_sum<NACC + 1>(v, row);
-        __syncthreads();
-        if (threadIdx.x <= NACC) xh[(size_t)threadIdx.x * P.cap + i] = row[threadIdx.x];
-        __syncthreads();
-    }
+    for (int h = blockIdx.x; h < nh; h += gridDim.x)
+        hard_row(jr, st, P.cap, P.eps2, hlist[h], 1, P.n, xh);
 }
 
 // Sum of the S chunk partials of one star.  `part` points at the
