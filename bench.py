@@ -123,13 +123,35 @@ def roofline(n, avg_ms, peak, traffic=None, valu=None):
     return out
 
 
-def load_valu(kind, shape):
-    """Measured VALU work per launch (rocprofv3 SQ_INSTS_VALU_* counters), if
-    profiled at this launch shape."""
+def sources_sha256():
+    """Hash of the library's sources (pn_amd/csrc/*), which every committed
+    counter summary records: counters measured on other sources are not
+    reported as this build's."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "pn_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".hpp", ".inc", ".h", ".c")):
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(name.encode() + b"\0" + f.read())
+    return h.hexdigest()
+
+
+def load_profile(name):
+    """A committed counter summary, if it was measured on these sources."""
     try:
-        with open(os.path.join(ROOT, "profiles", "pair_kernel_valu_n131073.json")) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             d = json.load(f)
     except (OSError, ValueError):
+        return None
+    return d if d.get("sources_sha256") == sources_sha256() else None
+
+
+def load_valu(kind, shape):
+    """Measured VALU work per launch (rocprofv3 SQ_INSTS_VALU_* counters), if
+    profiled on these sources at this launch shape."""
+    d = load_profile("pair_kernel_valu_n131073.json")
+    if d is None:
         return None
     want = d.get("shape", {}).get(kind, {})
     if any(shape.get(k) != v for k, v in want.items()):
@@ -139,13 +161,10 @@ def load_valu(kind, shape):
 
 def load_traffic(kind, name="pair_kernel_traffic_n131073.json", shape=None):
     """HBM bytes per pair launch from a committed rocprofv3 --pmc summary,
-    if it was measured at this launch shape (the partial sums written scale
-    with the j split / chunk length); else None."""
-    path = os.path.join(ROOT, "profiles", name)
-    try:
-        with open(path) as f:
-            d = json.load(f)
-    except (OSError, ValueError):
+    if it was measured on these sources at this launch shape (the partial
+    sums written scale with the j split / chunk length); else None."""
+    d = load_profile(name)
+    if d is None:
         return None
     if shape is not None and "chunk" in shape and "by_chunk" in d:
         row = d["by_chunk"].get(kind, {}).get(str(shape["chunk"]))

@@ -598,6 +598,21 @@ __device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict
 
 // Mixed-form j loop: PP bit 0 clear: one body at a time; set: two bodies
 // per iteration, records two ahead (as run64).
+// One j-record with a single 64-byte scalar load (one wait per pair instead
+// of one per group of fields the body reaches).
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+__device__ __forceinline__ JRec32 load_rec(const JRec32* __restrict__ p)
+{
+    const f32x16 v = *reinterpret_cast<const f32x16*>(p);
+    JRec32 r;
+    r.xh = v[0]; r.yh = v[1]; r.zh = v[2];
+    r.xl = v[3]; r.yl = v[4]; r.zl = v[5];
+    r.sx = v[6]; r.sy = v[7]; r.sz = v[8];
+    r.m = v[9]; r.A = v[10]; r.B = v[11]; r.MB = v[12];
+    r.Qdx = v[13]; r.Qdy = v[14]; r.Qdz = v[15];
+    return r;
+}
+
 // A wave-uniform pair counter folds the Newtonian sums into Kahan totals
 // every `tile` pairs without restarting the record stream.
 template <bool MASK, int PP, bool TS, bool EQ>
@@ -610,7 +625,8 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
     constexpr bool RF = (PP & 2) != 0;
     if ((PP & 1) == 0) {
         for (int j = j0; j < j1; ++j) {
-            body32<MASK, RF, TS, EQ>(jr[j], j64, j, i, I, a, sN, cN, eps2, eps2d, rscale);
+            body32<MASK, RF, TS, EQ>(load_rec(jr + j), j64, j, i, I, a, sN, cN, eps2, eps2d,
+                                     rscale);
             if (++cnt >= tile) {
                 cnt = 0;
                 fold_newton(a, sN, cN);

@@ -11,10 +11,13 @@ C4 size (N = 16384, 256 steps) the oracle is too slow, and the gradient is
 checked against central finite differences of GPU trajectories
 (h = 1e-6 relative), whose own truncation and rounding set the bound 1e-5.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle_bind as ob
+from conftest import GOLDEN
 from pn_amd import lib as pn
 from pn_amd import workloads
 
@@ -239,6 +242,37 @@ def test_adjoint_c4_vs_finite_differences(ctx):
     err = np.abs(got - want).max() / np.abs(want).max()
     print(f"C4: loss {loss:.6e}, |g| max {np.abs(want).max():.3e}, FD rel err {err:.3e}")
     assert err <= 1e-5, (got, want)
+
+
+def test_adjoint_c4_segment_vs_oracle_complex_step(ctx):
+    """The differentiation workload at full size (N = 16384 Plummer, seed 4,
+    + BH 0.02, c = 100, eps = 1e-3, dt = 2^-10) over 8 steps: the GPU adjoint
+    of the radial-velocity loss (r < 0.2) against complex step through the
+    oracle's own stage driver (oracle/ls9_oracle_cs.c) at m0 and 16
+    coordinates of (x0, v0), the committed fixture
+    tests/golden/c4_grad_8steps.npz (tests/golden/make_c4_grad.py): max
+    difference <= 1e-10 of the largest component."""
+    z = np.load(os.path.join(GOLDEN, "c4_grad_8steps.npz"))
+    W = workloads.ADJOINT
+    steps = int(z["steps"])
+    x, v, m = pn.ic_plummer(W["n_stars"], W["seed"], a=1.0, m_bh=W["m_bh"])
+    print(f"case: C4 segment, N = {W['n_stars']}, {steps} steps", flush=True)
+    ctx.set_params(W["eps"], W["c"], W["dt"], pn.FP64)
+    ctx.upload(x, v, m)
+    ctx.adjoint_forward(steps, 16)
+    xt, vt = ctx.download()
+    loss, lx, lv = workloads.radial_loss(xt, vt, W["r_cut"])
+    gx0, gv0, gm0 = ctx.adjoint_backward(lx, lv)
+    assert abs(loss - float(z["loss"])) <= 1e-12 * abs(float(z["loss"]))
+    names = {0: "m0", 1: "x", 2: "v"}
+    coords = [(names[int(k)],) if int(k) == 0 else (names[int(k)], int(p), int(a))
+              for k, p, a in zip(z["kind"], z["particle"], z["axis"])]
+    got = pick(gx0, gv0, gm0, coords)
+    want = z["grad"]
+    err = np.abs(got - want) / np.abs(want).max()
+    print(f"C4 segment: loss {loss:.9e}, |g| max {np.abs(want).max():.3e}, max diff "
+          f"{err.max():.3e} of it (coordinate {coords[int(err.argmax())]})", flush=True)
+    assert err.max() <= 1e-10, (got, want)
 
 
 def test_torch_autograd_wrapper(ctx):
