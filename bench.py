@@ -327,7 +327,7 @@ def run_ensemble(steps, warmup, chunk=0, tile=0, variant=-1, eager=False, adj_st
             ens.set_params(W["eps"], W["c"], W["dt"], mode)
             ens.upload(cl)
             if autotune:
-                ens.autotune(2)                     # before the timed region
+                ens.autotune(4)                     # before the timed region
             shapes[name] = ens.get_tuning()
             e0 = ens.energy()
             wall = time_steps(ens, steps, warmup, eager)
