@@ -45,8 +45,11 @@ struct pn1_ens {
     // FP64 sums of their pairs [NACC + 1][cap]
     int *fl = nullptr, *fblk = nullptr, *flist = nullptr, *fn = nullptr;
     double* xp = nullptr;
-    // hard stars (PN1_HARD_K): per-cluster lists and FP64 rows [NACC + 1][cap]
+    // hard stars (PN1_HARD_K): per-cluster lists, all clusters' (star,
+    // cluster) pairs [cap] (their number in counter[NXCD]) and FP64 rows
+    // [NACC + 1][cap]
     int* hlist = nullptr;
+    int2* hall = nullptr;
     double* xh = nullptr;
     int fblk_cap = 0, gblk_cap = 0, bhblk_cap = 0, eblk_cap = 0;
     pn1dev::JRec64* j64 = nullptr;

@@ -7,7 +7,8 @@
 //   k_ens_prep   per-star j-records and per-block separable sums; blocks
 //                come from a block table, each inside one cluster
 //   k_ens_gsum   one workgroup per cluster: that cluster's separable sums;
-//                also resets the work counters of the pair kernel
+//                also resets the work counters of the pair kernel and the
+//                count of the mixed form's hard stars
 //   k_ens_pair*  ONE persistent launch for all clusters: workgroups take
 //                (cluster, 256-star i-block, j-chunk) items from per-XCD
 //                queues (atomic counters), items sorted by decreasing cost
@@ -220,38 +221,46 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
 
 // One workgroup per cluster: its BH-close stars, in increasing order, at
 // flist[coff[c] ..), their number in fn[c]; its hard stars likewise at
-// hlist[coff[c] ..), their number in hn[c].
+// hlist[coff[c] ..), their number in hn[c], and appended as (star,
+// cluster) pairs to the evaluation's list of every cluster's hard stars,
+// hall[0 .. *hcount) (k_ens_gsum zeroes *hcount).  The clusters' order in
+// hall depends on which workgroup reserves its range first; each hard row
+// is computed on its own, so the results do not.
 __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
                                                  const int* __restrict__ fblk,
                                                  const int* __restrict__ cblk,
                                                  const int* __restrict__ coff,
                                                  int* __restrict__ flist, int* __restrict__ fn,
-                                                 int* __restrict__ hlist, int* __restrict__ hn)
+                                                 int* __restrict__ hlist, int* __restrict__ hn,
+                                                 int2* __restrict__ hall,
+                                                 unsigned int* __restrict__ hcount)
 {
+    __shared__ int s_at;
     const int c = blockIdx.x;
     list_flagged<0, 2>(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1, coff[c + 1],
                        flist + coff[c], fn + c);
-    list_flagged<1, 2>(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1, coff[c + 1],
-                       hlist + coff[c], hn + c);
+    const int nh = list_flagged<1, 2>(fl, fblk, cblk[c], cblk[c + 1], coff[c] + 1,
+                                      coff[c + 1], hlist + coff[c], hn + c);
+    if (nh == 0) return;                     // uniform: every thread holds the count
+    if (threadIdx.x == 0) s_at = (int)atomicAdd(hcount, (unsigned int)nh);
+    __syncthreads();                         // s_at, and the list written above
+    for (int k = (int)threadIdx.x; k < nh; k += WG)
+        hall[s_at + k] = make_int2(hlist[coff[c] + k], c);
 }
 
 // FP64 rows of every cluster's hard stars (as k_hard_rows, j over the
-// star's own cluster): the workgroups take the concatenated lists in turn.
+// star's own cluster): the workgroups take the entries of hall in turn.
 __global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
                                                       const double* __restrict__ st,
-                                                      const int* __restrict__ coff, int nc,
-                                                      const int* __restrict__ hlist,
-                                                      const int* __restrict__ hn,
+                                                      const int* __restrict__ coff,
+                                                      const int2* __restrict__ hall,
+                                                      const unsigned int* __restrict__ hcount,
                                                       double* __restrict__ xh, Phys P)
 {
-    const int G = (int)gridDim.x;
-    int base = 0;
-    for (int c = 0; c < nc; ++c) {
-        const int nh = hn[c];
-        const int off = coff[c], end = coff[c + 1];
-        for (int h = (((int)blockIdx.x - base) % G + G) % G; h < nh; h += G)
-            hard_row(jr, st, P.cap, P.eps2, hlist[off + h], off + 1, end, xh);
-        base += nh;
+    const int nh = (int)*hcount;
+    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
+        const int2 w = hall[h];
+        hard_row(jr, st, P.cap, P.eps2, w.x, coff[w.y] + 1, coff[w.y + 1], xh);
     }
 }
 
@@ -293,7 +302,7 @@ __global__ __launch_bounds__(WG) void k_ens_gsum(const double* __restrict__ gblk
                                                 unsigned int* __restrict__ counter)
 {
     const int c = blockIdx.x;
-    if (c == 0 && threadIdx.x < NXCD) counter[threadIdx.x] = 0u;
+    if (c == 0 && threadIdx.x <= NXCD) counter[threadIdx.x] = 0u;   // queues, hard stars
     double g[NGS];
     cluster_rows_sum<NGS>(gblk, cblk, c, g);
     block_sum<NGS>(g, gs + (size_t)c * NGS);

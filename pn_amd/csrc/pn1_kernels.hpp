@@ -216,9 +216,9 @@ __device__ __forceinline__ int block_excl_scan(int v, int* total)
 // The stars with bit BIT of fl set in blocks [b0, b1) (block b covers the
 // stars p0 + (b - b0) * WG .. + WG - 1, its count of them at
 // fblk[b * STRIDE + BIT]) listed in increasing order at out[0 ..), their
-// number in *nout.  One workgroup.
+// number in *nout and returned to every thread.  One workgroup.
 template <int BIT = 0, int STRIDE = 1>
-__device__ __forceinline__ void list_flagged(const int* __restrict__ fl,
+__device__ __forceinline__ int list_flagged(const int* __restrict__ fl,
                                              const int* __restrict__ fblk, int b0, int b1,
                                              int p0, int pend, int* __restrict__ out,
                                              int* __restrict__ nout)
@@ -230,14 +230,22 @@ __device__ __forceinline__ void list_flagged(const int* __restrict__ fl,
         int total;
         const int at = base + block_excl_scan(cnt, &total);
         if (cnt > 0) {
+            // the block's flags 16 at a time: 16 loads in flight, not one
             int k = at;
-            const int q0 = p0 + (b - b0) * WG;
-            for (int q = q0; q < q0 + WG && q < pend; ++q)
-                if ((fl[q] >> BIT) & 1) out[k++] = q;
+            const int q0 = p0 + (b - b0) * WG, qe = min(q0 + WG, pend);
+            for (int q = q0; q < qe; q += 16) {
+                int f[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) f[u] = fl[min(q + u, qe - 1)];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (q + u < qe && ((f[u] >> BIT) & 1)) out[k++] = q + u;
+            }
         }
         base += total;
     }
     if (threadIdx.x == 0) *nout = base;
+    return base;
 }
 
 // The BH-close stars (flist, fn[0]) and the hard stars (hlist, fn[1]) of

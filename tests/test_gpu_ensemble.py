@@ -176,6 +176,69 @@ def test_mixed_accuracy(ens, clusters):
         assert err[1:].max() <= 1e-6 and err[0] <= 1e-6, (k, err[1:].max(), err[0])
 
 
+HARD_K = 30.0          # PN1_HARD_K: a star whose sum m_j w^2 exceeds 30 |a| gets an FP64 row
+
+
+def lattice_cluster(q, h, seed, bh_dist=40.0):
+    """(2q+1)^3 equal stars on a jittered cubic lattice of spacing h around
+    a star at its exact centre; the BH (mass 0.01) bh_dist away.  The
+    centre star's star terms cancel to its neighbours' jitter and the far
+    BH pulls little: its terms cancel far beyond HARD_K."""
+    rng = np.random.default_rng(seed)
+    g = np.arange(-q, q + 1) * h
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = pts[np.argsort(np.abs(pts).sum(1), kind="stable")]     # centre first
+    pts[1:] += rng.uniform(-0.02 * h, 0.02 * h, pts[1:].shape)
+    ns = len(pts)
+    x = np.vstack([[bh_dist, 0.0, 0.0], pts])
+    v = np.vstack([np.zeros(3), rng.normal(0, 0.2, (ns, 3))])
+    m = np.concatenate([[0.01], np.full(ns, 1.0 / ns)])
+    return x, v, m
+
+
+def cancel_ratio(x, m, a):
+    """sum_j m_j / (r_ij^2 + eps^2) over the other stars, over |a_i|, per star."""
+    xs = x[1:]
+    d = xs[None, :, :] - xs[:, None, :]
+    w2 = 1.0 / ((d * d).sum(-1) + EPS * EPS)
+    np.fill_diagonal(w2, 0.0)
+    return (w2 * m[None, 1:]).sum(1) / np.linalg.norm(a[1:], axis=1)
+
+
+def test_hard_stars_of_many_clusters(ens, ctx, clusters):
+    """Stars whose pair terms cancel (HARD_K) take FP64 rows of their own;
+    the ensemble lists them for all its clusters in one pass.  Five lattice
+    clusters, each with such stars, between ordinary ones: the mixed forces
+    of every star are within 1e-6 of FP64, and with one chunk per cluster
+    (an unsplit single context) bitwise the single-cluster results."""
+    lat = [lattice_cluster(3, 0.1 + 0.01 * k, 60 + k) for k in range(5)]
+    mix = [clusters[8], lat[0], clusters[10], lat[1], lat[2], clusters[12], lat[3], lat[4]]
+    ens.set_tuning(4096, 0, -1)
+    ens.set_params(EPS, C, DT, pn.FP64)
+    ens.upload(mix)
+    a64 = ens.force()
+    for k, (x, _, m) in enumerate(mix):
+        if k in (1, 3, 4, 6, 7):
+            kr = cancel_ratio(x, m, a64[k])
+            print(f"lattice cluster {k}: {int((kr > HARD_K).sum())} stars with "
+                  f"sum m w^2 / |a| > {HARD_K:g} (max {kr.max():.3e})", flush=True)
+            assert (kr > 3 * HARD_K).any()
+    ens.set_params(EPS, C, DT, pn.MIXED)
+    ens.upload(mix)
+    ens.force()                       # thresholds and hard flags from this one
+    a32 = ens.force()
+    for k, cl in enumerate(mix):
+        err = rel_err(a32[k], a64[k])
+        print(f"cluster {k} n={len(cl[2])}: mixed vs FP64 star max {err[1:].max():.3e} "
+              f"BH {err[0]:.3e}", flush=True)
+        assert err[1:].max() <= 1e-6 and err[0] <= 1e-6, (k, err[1:].max(), err[0])
+        ctx.set_tuning(1, 0, -1)
+        ctx.set_params(EPS, C, DT, pn.MIXED)
+        ctx.upload(*cl)
+        ctx.force()
+        assert np.array_equal(a32[k], ctx.force()), k
+
+
 def test_graph_replay_equals_eager_profiled(ens, clusters):
     """pn1_ens_step replays a captured graph; with profiling on it launches
     eagerly.  Both must give the same bits."""
