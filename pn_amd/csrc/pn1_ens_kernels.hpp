@@ -340,7 +340,7 @@ __global__ __launch_bounds__(WG) void k_ens_fin(const double* in, double* X, dou
             const void* base = MODE == 0
                                    ? (const void*)((const double*)part + poff[b.c] * NACC)
                                    : (const void*)((const float*)part + poff[b.c] * MROWS);
-            gather_acc<MODE>(base, ens_nchunks(nst, E.L), (size_t)nst, li,
+            gather_acc<MODE, (STAGE <= 1)>(base, ens_nchunks(nst, E.L), (size_t)nst, li,
                              MODE == 1 ? cmmax[b.c] : 1.0, S_, ts);
             if (MODE == 1 && fn[b.c] > 0) add_flagged(xp, (size_t)P.cap, off + 1 + li, S_, ts);
         }

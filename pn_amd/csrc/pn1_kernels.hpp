@@ -912,8 +912,9 @@ __global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr
 // star's index in it.  FP64: [S][NACC][stride]; mixed: [S][MROWS][stride]
 // floats (the NACC32 sums, then the Newtonian Kahan compensations), whose
 // sums built from the record's mass m_j / mscale are scaled back by mscale;
-// ts is the mixed form's A_TS sum (0 for FP64).
-template <int MODE>
+// ts is the mixed form's A_TS sum (0 for FP64, and without TS: the stages
+// whose pair kernels do not write that row).
+template <int MODE, bool TS = true>
 __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S, size_t stride,
                                            int idx, double mscale, double (&A)[NACC], double& ts)
 {
@@ -931,7 +932,7 @@ __device__ __forceinline__ void gather_acc(const void* __restrict__ part, int S,
             const float* ps = pp + (size_t)s * MROWS * stride + idx;
 #pragma unroll
             for (int k = 0; k < NACC; ++k) A[k] += (double)ps[(size_t)k * stride];
-            ts += (double)ps[(size_t)A_TS * stride];
+            if (TS) ts += (double)ps[(size_t)A_TS * stride];
 #pragma unroll
             for (int k = 0; k < 3; ++k) A[A_NX + k] -= (double)ps[(size_t)(NACC32 + k) * stride];
         }
@@ -1094,7 +1095,7 @@ __global__ __launch_bounds__(WG) void k_fin(const double* in, double* X, double*
         if (MODE == 1 && (fl[p] & 2)) {
             load_row(xh, (size_t)P.cap, p, S_, ts);
         } else {
-            gather_acc<MODE>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
+            gather_acc<MODE, (STAGE <= 1)>(part, S, (size_t)P.cap, p, P.mmax, S_, ts);
             if (MODE == 1 && fn[0] > 0) add_flagged(xp, (size_t)P.cap, p, S_, ts);
         }
         fin_star<STAGE>(in, X, Y, H1, H2, m, P.cap, p, 0, S_, gs, fin_consts(P, m[0]), acc, cb, a);
