@@ -201,9 +201,25 @@ __device__ __forceinline__ void stream_out(T v, T* p)
     __builtin_nontemporal_store(v, p);
 }
 
+// Tapered j split.  A split in S >= 2 PN1_TAPER_N chunks (S a multiple of
+// 8) ends in PN1_TAPER_N short chunks, each 1 / PN1_TAPER_F as long as the
+// others, and a pair launch runs the long chunks of all its i blocks before
+// any short one.  Equal workgroups leave the chip idling at the end of a
+// launch for about half a workgroup's duration, waiting for the last ones;
+// with short workgroups last, that wait shrinks in proportion (measured on
+// MI355X at n = 131073, S = 64: ~1.5% shorter mixed and FP64 steps).
+#define PN1_TAPER_N 16
+#define PN1_TAPER_F 8
+
+__device__ __forceinline__ bool tapered(int S)
+{
+    return S >= 2 * PN1_TAPER_N && (S & 7) == 0;
+}
+
 // Workgroup -> (j chunk s, i block ib).  When S divides 8, consecutive
 // workgroup ids (which the dispatcher deals round-robin to the 8 XCDs) are
-// arranged so that every workgroup on XCD x works on chunk x % S.
+// arranged so that every workgroup on XCD x works on chunk x % S; for
+// larger S (multiples of 8) the chunk is bid % S, also x modulo 8.
 __device__ __forceinline__ void map_block(int bid, int S, int& s, int& ib)
 {
     if (S <= 8 && (8 % S) == 0) {
@@ -216,7 +232,29 @@ __device__ __forceinline__ void map_block(int bid, int S, int& s, int& ib)
     }
 }
 
-// Chunk bounds of the j range [1, n) split in S parts, and the part of it
+// The same for a launch over nib i blocks with the tapered split: first
+// the S - PN1_TAPER_N long chunks of every block, then the short ones
+// (both parts keep chunk = XCD modulo 8).
+__device__ __forceinline__ void map_block(int bid, int S, int nib, int& s, int& ib)
+{
+    if (!tapered(S)) {
+        map_block(bid, S, s, ib);
+        return;
+    }
+    const int Sm = S - PN1_TAPER_N, nm = Sm * nib;
+    if (bid < nm) {
+        s = bid % Sm;
+        ib = bid / Sm;
+    } else {
+        const int t = bid - nm;
+        s = Sm + t % PN1_TAPER_N;
+        ib = t / PN1_TAPER_N;
+    }
+}
+
+// Chunk bounds of the j range [1, n) split in S parts (tapered: S - N long
+// chunks of length cm, N short ones of ceil(cm / F), (S - N + N / F) cm >=
+// n - 1; the last short chunks may be empty), and the part of the chunk
 // that can contain j == i for this wave's 64 lanes.
 struct JRange {
     int j0, j1, w0, w1;
@@ -225,9 +263,19 @@ __device__ __forceinline__ JRange j_range(int s, int S, int n, int ib)
 {
     JRange R;
     const int nst = n - 1;
-    const int clen = (nst + S - 1) / S;
-    R.j0 = 1 + s * clen;
-    R.j1 = min(n, R.j0 + clen);
+    if (!tapered(S)) {
+        const int clen = (nst + S - 1) / S;
+        R.j0 = 1 + s * clen;
+        R.j1 = min(n, R.j0 + clen);
+    } else {
+        constexpr int eq = PN1_TAPER_N / PN1_TAPER_F;   // short chunks per long one
+        const int Sm = S - PN1_TAPER_N;
+        const int cm = (nst + Sm + eq - 1) / (Sm + eq);
+        const int ct = (cm + PN1_TAPER_F - 1) / PN1_TAPER_F;
+        const int a = s < Sm ? 1 + s * cm : 1 + Sm * cm + (s - Sm) * ct;
+        R.j0 = min(n, a);
+        R.j1 = min(n, a + (s < Sm ? cm : ct));
+    }
     const int wb = ib * WG + (__builtin_amdgcn_readfirstlane((int)threadIdx.x) & ~63) + 1;
     R.w0 = max(R.j0, wb);
     R.w1 = min(R.j1, wb + 64);

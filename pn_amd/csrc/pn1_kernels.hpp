@@ -376,7 +376,7 @@ __global__ __launch_bounds__(WG) void k_pair64(const JRec64* __restrict__ jr,
                                               int nIblk, int ib0)
 {
     int s, ib;
-    map_block(blockIdx.x, S, s, ib);
+    map_block(blockIdx.x, S, nIblk, s, ib);
     if (ib >= nIblk) return;
     ib += ib0;
     const int n = P.n;
@@ -678,7 +678,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
                                               int nIblk, int ib0, int tile)
 {
     int s, ib;
-    map_block(blockIdx.x, S, s, ib);
+    map_block(blockIdx.x, S, nIblk, s, ib);
     if (ib >= nIblk) return;
     ib += ib0;
     const int n = P.n;
@@ -743,11 +743,10 @@ void k_pair32q(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64,
     const int li = ib * WG + iw * 64 + lane + 1;
     const int i = li < n ? li : n - 1;
     const I32 I = load_i32(j64, thr, st, P.cap, i, P.eps2);
-    JRange R;
+    // chunk 4 s4 + sub of the S-chunk split (k_pair32's bounds), and the
+    // self window of this wave's 64 stars
+    JRange R = j_range(s4 * QSUB + sub, S, n, ib);
     {
-        const int nst = n - 1, clen = (nst + S - 1) / S, sc = s4 * QSUB + sub;
-        R.j0 = 1 + sc * clen;
-        R.j1 = min(n, R.j0 + clen);
         const int wb = ib * WG + iw * 64 + 1;
         R.w0 = max(R.j0, wb);
         R.w1 = min(R.j1, wb + 64);
@@ -1187,7 +1186,7 @@ __global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
                                                int nIblk)
 {
     int s, ib;
-    map_block(blockIdx.x, S, s, ib);
+    map_block(blockIdx.x, S, nIblk, s, ib);
     if (ib >= nIblk) return;
     const int n = P.n, cap = P.cap;
     const int li = ib * WG + threadIdx.x + 1;

@@ -11,10 +11,11 @@ Tolerances:
     is then compared with an absolute bound of a few ulp of those terms.
   FP64 form, energies: energy_atol (self-pair rounding of the reference).
   Mixed form: against the FP64 form, per-particle relative error
-    p99.9 <= 1e-6 and max <= 1e-5 (stars), BH <= 1e-6, for c >= 50.  At
-    c = 1 (the reference's own units) the 1PN three-body terms are as large
-    as the Newtonian ones; the mixed form compensates only its Newtonian
-    sums (the 1PN sums enter scaled by 1/c^2), and the bound is p99.9 <= 5e-6.
+    max <= 1e-6 for every star and the BH, for c >= 50.  At c = 1 (the
+    reference's own units) the 1PN three-body terms are as large as the
+    Newtonian ones; the mixed form compensates only its Newtonian sums (the
+    1PN sums enter scaled by 1/c^2), and the star bound is p99.9 <= 5e-6,
+    max <= 1e-5 (the BH's stays 1e-6).
 """
 import numpy as np
 import pytest
@@ -104,6 +105,34 @@ def test_fp64_force_vs_oracle_larger(ctx):
         print(f"n=4097 c={c}: star max {err[1:].max():.3e} BH {err[0]:.3e}")
         assert err[1:].max() <= 1e-12
         assert err[0] <= 1e-12
+
+
+@pytest.mark.parametrize("n_stars", [39, 99, 256, 1000])
+def test_tapered_split_small_n(ctx, n_stars):
+    """Splits of 32 chunks and more end in 16 short chunks (a quarter ...
+    an eighth of the others' length), which at these sizes are one record
+    long or empty.  Every split must still cover each pair once: FP64 rows
+    against the oracle to 1e-12 and the mixed form within 1e-6 of FP64, for
+    untapered (8, 24) and tapered (32, 64, 128) splits alike."""
+    x, v, m = pn.ic_plummer(n_stars, 90 + n_stars, a=1.0, m_bh=0.02)
+    ref = ob.orc_force(x, v, m, 1e-3, 100.0)
+    try:
+        for split in (8, 24, 32, 64, 128):
+            ctx.set_tuning(split, 0, -1)
+            ctx.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+            ctx.upload(x, v, m)
+            assert ctx.get_tuning()[0] == min(split, n_stars)    # at most one chunk per star
+            a64 = ctx.force()
+            err = rel_err(a64, ref)
+            ctx.set_params(1e-3, 100.0, 2.0**-10, pn.MIXED)
+            ctx.upload(x, v, m)
+            e32 = rel_err(ctx.force(), a64)
+            print(f"n={n_stars + 1} split {split}: FP64 vs oracle {err.max():.3e}, "
+                  f"mixed vs FP64 {e32.max():.3e}", flush=True)
+            assert err.max() <= 1e-12, (split, err.max())
+            assert e32.max() <= 1e-6, (split, e32.max())
+    finally:
+        ctx.set_tuning(0, 0, -1)
 
 
 @pytest.mark.parametrize("name", ["traj_rand_n65_c1", "traj_c1_n1025_c100"])
