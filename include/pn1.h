@@ -144,7 +144,10 @@ int  pn1_sync(pn1_ctx *ctx);
 const char *pn1_last_error(const pn1_ctx *ctx);
 
 /* Launch shape of the pair kernels: the j range is split into `jsplit`
- * chunks (0 = automatic), the mixed kernel compensates its FP32 sums every
+ * chunks (0 = automatic; a split of 32 or more, a multiple of 8, ends in 16
+ * chunks an eighth as long as the others, which the launch runs last, so
+ * that it does not wait long on its last workgroups), the mixed kernel
+ * compensates its FP32 sums every
  * `tile` pairs (0 = default), and `variant` picks the kernel form (-1 =
  * automatic per mode): bit 0 the j loop (clear: one j-record at a time,
  * set: two in flight), bit 1 (mixed form) one Newton step on the FP32
