@@ -304,7 +304,10 @@ void launch_prep(pn1_ctx* c, const double* in, const Phys& P)
                            in, (const int*)c->flist, (const int*)c->fn, c->xp, P);
         hipLaunchKernelGGL(k_hard_rows, dim3(c->hgrid), dim3(HWG), 0, c->st,
                            (const JRec64*)c->j64, in, (const int*)c->hlist, (const int*)c->fn,
-                           c->xh, P);
+                           c->xh, c->hseg, P);
+        hipLaunchKernelGGL(k_hard_sum, dim3((HCAP * (NACC + 1) + WG - 1) / WG), dim3(WG), 0,
+                           c->st, (const int*)c->hlist, (const int*)c->fn,
+                           (const double*)c->hseg, c->xh, c->cap);
     }
 }
 
@@ -452,7 +455,7 @@ void pn1_destroy(pn1_ctx* c)
     if (c->j32) (void)hipFree(c->j32);
     if (c->thr) (void)hipFree(c->thr);
     if (c->kr) (void)hipFree(c->kr);
-    void* fbufs[] = {c->fl, c->fblk, c->flist, c->fn, c->xp, c->hlist, c->xh};
+    void* fbufs[] = {c->fl, c->fblk, c->flist, c->fn, c->xp, c->hlist, c->xh, c->hseg};
     for (void* b : fbufs)
         if (b) (void)hipFree(b);
     if (c->part) (void)hipFree(c->part);
@@ -508,6 +511,7 @@ int pn1_create(pn1_ctx** out, int capacity, int device)
     if (!rc) rc = dalloc(c, &c->flist, cap);
     if (!rc) rc = dalloc(c, &c->hlist, cap);
     if (!rc) rc = dalloc(c, &c->xh, (size_t)(NACC + 1) * cap);
+    if (!rc) rc = dalloc(c, &c->hseg, (size_t)HCAP * HSEG * (NACC + 1));
     if (!rc) rc = dalloc(c, &c->fn, 4);
     if (!rc) rc = dalloc(c, &c->xp, (size_t)(NACC + 1) * cap);
     if (!rc && host_malloc((void**)&c->h_stage, 7 * cap * sizeof(double)) != hipSuccess) {

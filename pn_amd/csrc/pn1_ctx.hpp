@@ -38,11 +38,13 @@ struct pn1_ctx {
     // their list and number) and the FP64 sums of their pairs
     int *fl = nullptr, *fblk = nullptr, *flist = nullptr, *fn = nullptr;
     double* xp = nullptr;
-    // hard stars (PN1_HARD_K): their list and FP64 rows [NACC + 1][cap];
-    // hgrid workgroups of k_hard_rows take the listed stars in turn
+    // hard stars (PN1_HARD_K): their list, FP64 rows [NACC + 1][cap] and the
+    // segment sums of the split rows [HCAP][HSEG][NACC + 1]; hgrid
+    // workgroups of k_hard_rows take the work units in turn
     int* hlist = nullptr;
     double* xh = nullptr;
-    int hgrid = 256;
+    double* hseg = nullptr;
+    int hgrid = 1024;
     // acc and kr hold the last mixed evaluation of the uploaded run (cleared
     // by pn1_upload and by FP64 evaluations): the mixed form's thresholds
     // read them

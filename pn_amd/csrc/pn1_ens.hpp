@@ -46,11 +46,13 @@ struct pn1_ens {
     int *fl = nullptr, *fblk = nullptr, *flist = nullptr, *fn = nullptr;
     double* xp = nullptr;
     // hard stars (PN1_HARD_K): per-cluster lists, all clusters' (star,
-    // cluster) pairs [cap] (their number in counter[NXCD]) and FP64 rows
-    // [NACC + 1][cap]
+    // cluster) pairs [cap] (their number in counter[NXCD]), FP64 rows
+    // [NACC + 1][cap] and the segment sums of the split rows
+    // [HCAP][HSEG][NACC + 1]
     int* hlist = nullptr;
     int2* hall = nullptr;
     double* xh = nullptr;
+    double* hseg = nullptr;
     int fblk_cap = 0, gblk_cap = 0, bhblk_cap = 0, eblk_cap = 0;
     pn1dev::JRec64* j64 = nullptr;
     pn1dev::JRec32* j32 = nullptr;

@@ -249,19 +249,33 @@ __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
 }
 
 // FP64 rows of every cluster's hard stars (as k_hard_rows, j over the
-// star's own cluster): the workgroups take the entries of hall in turn.
+// star's own cluster): the first min(*hcount, HCAP) entries of hall in
+// HSEG units each (into hseg), the rest whole; the workgroups take the
+// units in turn.
 __global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
                                                       const double* __restrict__ st,
                                                       const int* __restrict__ coff,
                                                       const int2* __restrict__ hall,
                                                       const unsigned int* __restrict__ hcount,
-                                                      double* __restrict__ xh, Phys P)
+                                                      double* __restrict__ xh,
+                                                      double* __restrict__ hseg, Phys P)
 {
-    const int nh = (int)*hcount;
-    for (int h = blockIdx.x; h < nh; h += gridDim.x) {
-        const int2 w = hall[h];
-        hard_row(jr, st, P.cap, P.eps2, w.x, coff[w.y] + 1, coff[w.y + 1], xh);
+    const int nh = (int)*hcount, ns = min(nh, HCAP), nu = ns * HSEG + (nh - ns);
+    for (int u = blockIdx.x; u < nu; u += gridDim.x) {
+        const int2 w = hall[u < ns * HSEG ? u / HSEG : ns + (u - ns * HSEG)];
+        hard_unit(jr, st, P.cap, P.eps2, u, ns, w.x, coff[w.y] + 1, coff[w.y + 1], xh, hseg);
     }
+}
+
+// The split rows' segment sums, in segment order (as k_hard_sum).
+__global__ __launch_bounds__(WG) void k_ens_hard_sum(const int2* __restrict__ hall,
+                                                    const unsigned int* __restrict__ hcount,
+                                                    const double* __restrict__ hseg,
+                                                    double* __restrict__ xh, int cap)
+{
+    const int t = blockIdx.x * WG + threadIdx.x;
+    const int ns = min((int)*hcount, HCAP);
+    if (t < ns * (NACC + 1)) hard_sum(hseg, t, hall[t / (NACC + 1)].x, cap, xh);
 }
 
 // Per star block: its stars' FP64 pairs with their cluster's flagged stars

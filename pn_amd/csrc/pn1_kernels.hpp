@@ -842,16 +842,27 @@ __global__ __launch_bounds__(WG) void k_pair_fl(const JRec64* __restrict__ jr,
     flagged_pairs(jr, st, P.cap, i, P.eps2, flist, nf, xp, (size_t)P.cap, i);
 }
 
-// FP64 row of hard star i (see PN1_HARD_K) over the records [j0, j1) with
-// the FP64 form's body (BH-close stars included, with their full records):
-// the NACC sums and sum m_j w^2 into xh [NACC + 1][cap] at i.  One
-// workgroup of HWG threads per row, two records per lane in flight (the
-// loads of a row are not shared by other lanes, so their latency is hidden
-// only by having many in flight).
-constexpr int HWG = 1024;
+// FP64 sums of hard star i (see PN1_HARD_K) over the records [j0, j1)
+// with the FP64 form's body (BH-close stars included, with their full
+// records): the NACC sums and sum m_j w^2, reduced over the workgroup's HWG
+// lanes in a fixed order and written to out[k * stride + idx], k <= NACC.
+// Two records per lane in flight (the loads of a row are not shared by
+// other lanes, so their latency is hidden only by having many in flight).
+constexpr int HWG = 256;
+// A hard star's row is split in HSEG segments of the j range, each a work
+// unit of its own: one workgroup streaming a whole row (16 MB of records
+// at n = 131073) took ~460 us on one CU, on the critical path of every
+// mixed evaluation; 32 units on 32 CUs take ~1/32 of that.  The segments'
+// sums go to a scratch of HCAP stars x HSEG x (NACC + 1) and are added in
+// segment order (k_hard_sum); stars beyond the first HCAP of the list take
+// their row in one unit.
+constexpr int HSEG = 32;
+constexpr int HCAP = 1024;
+
 __device__ __forceinline__ void hard_row(const JRec64* __restrict__ jr,
                                          const double* __restrict__ st, int cap, double eps2,
-                                         int i, int j0, int j1, double* __restrict__ xh)
+                                         int i, int j0, int j1, double* __restrict__ out,
+                                         size_t stride, size_t idx)
 {
     __shared__ double wsum[NACC + 1][HWG / 64];
     const I64 I = load_i64(jr, st, cap, i, eps2);
@@ -889,21 +900,64 @@ __device__ __forceinline__ void hard_row(const JRec64* __restrict__ jr,
         double t = 0.0;
 #pragma unroll
         for (int w = 0; w < HWG / 64; ++w) t += wsum[threadIdx.x][w];
-        xh[(size_t)threadIdx.x * cap + i] = t;
+        out[(size_t)threadIdx.x * stride + idx] = t;
     }
     __syncthreads();
 }
 
-// The hard stars hlist[0 .. fn[1]), one workgroup per star at a time.
+// Work unit u of the rows of nh hard stars of one cluster whose stars are
+// [p0, pend): the first min(nh, HCAP) stars in HSEG segments each (unit
+// h * HSEG + g: star h, segment g, into hseg), the others whole (into xh).
+__device__ __forceinline__ void hard_unit(const JRec64* __restrict__ jr,
+                                          const double* __restrict__ st, int cap, double eps2,
+                                          int u, int ns, int star, int p0, int pend,
+                                          double* __restrict__ xh, double* __restrict__ hseg)
+{
+    if (u < ns * HSEG) {
+        const int g = u % HSEG, len = (pend - p0 + HSEG - 1) / HSEG;
+        const int j0 = min(pend, p0 + g * len), j1 = min(pend, j0 + len);
+        hard_row(jr, st, cap, eps2, star, j0, j1, hseg + (size_t)u * (NACC + 1), 1, 0);
+    } else {
+        hard_row(jr, st, cap, eps2, star, p0, pend, xh, (size_t)cap, (size_t)star);
+    }
+}
+
+// The hard stars hlist[0 .. fn[1]): ns = min(fn[1], HCAP) of them in HSEG
+// units each, the rest one unit each; the workgroups take the units in turn.
 __global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr,
                                                   const double* __restrict__ st,
                                                   const int* __restrict__ hlist,
                                                   const int* __restrict__ fn,
-                                                  double* __restrict__ xh, Phys P)
+                                                  double* __restrict__ xh,
+                                                  double* __restrict__ hseg, Phys P)
 {
-    const int nh = fn[1];
-    for (int h = blockIdx.x; h < nh; h += gridDim.x)
-        hard_row(jr, st, P.cap, P.eps2, hlist[h], 1, P.n, xh);
+    const int nh = fn[1], ns = min(nh, HCAP), nu = ns * HSEG + (nh - ns);
+    for (int u = blockIdx.x; u < nu; u += gridDim.x) {
+        const int h = u < ns * HSEG ? u / HSEG : ns + (u - ns * HSEG);
+        hard_unit(jr, st, P.cap, P.eps2, u, ns, hlist[h], 1, P.n, xh, hseg);
+    }
+}
+
+// Row value k of split hard star h (t = h * (NACC + 1) + k, h < ns): its
+// HSEG segment sums added in segment order, into xh at the star.
+__device__ __forceinline__ void hard_sum(const double* __restrict__ hseg, int t, int star,
+                                         int cap, double* __restrict__ xh)
+{
+    const int h = t / (NACC + 1), k = t % (NACC + 1);
+    const double* q = hseg + (size_t)h * HSEG * (NACC + 1) + k;
+    double s = 0.0;
+    for (int g = 0; g < HSEG; ++g) s += q[(size_t)g * (NACC + 1)];
+    xh[(size_t)k * cap + star] = s;
+}
+
+__global__ __launch_bounds__(WG) void k_hard_sum(const int* __restrict__ hlist,
+                                                const int* __restrict__ fn,
+                                                const double* __restrict__ hseg,
+                                                double* __restrict__ xh, int cap)
+{
+    const int t = blockIdx.x * WG + threadIdx.x;
+    const int ns = min(fn[1], HCAP);
+    if (t < ns * (NACC + 1)) hard_sum(hseg, t, hlist[t / (NACC + 1)], cap, xh);
 }
 
 // Sum of the S chunk partials of one star.  `part` points at the
