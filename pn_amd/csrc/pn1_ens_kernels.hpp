@@ -249,9 +249,10 @@ __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
 }
 
 // FP64 rows of every cluster's hard stars (as k_hard_rows, j over the
-// star's own cluster): the first min(*hcount, HCAP) entries of hall in
-// HSEG units each (into hseg), the rest whole; the workgroups take the
-// units in turn.
+// star's own cluster): every entry of hall in HSEG units (into hseg), or,
+// beyond HCAP entries, every one whole; the workgroups take the units in
+// turn.  With at most HCAP hard stars in the ensemble, each row is summed
+// as a single-cluster run sums it.
 __global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
                                                       const double* __restrict__ st,
                                                       const int* __restrict__ coff,
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict_
                                                       double* __restrict__ xh,
                                                       double* __restrict__ hseg, Phys P)
 {
-    const int nh = (int)*hcount, ns = min(nh, HCAP), nu = ns * HSEG + (nh - ns);
+    const int nh = (int)*hcount, ns = hard_split(nh), nu = ns * HSEG + (nh - ns);
     for (int u = blockIdx.x; u < nu; u += gridDim.x) {
         const int2 w = hall[u < ns * HSEG ? u / HSEG : ns + (u - ns * HSEG)];
         hard_unit(jr, st, P.cap, P.eps2, u, ns, w.x, coff[w.y] + 1, coff[w.y + 1], xh, hseg);
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(WG) void k_ens_hard_sum(const int2* __restrict__ ha
                                                     double* __restrict__ xh, int cap)
 {
     const int t = blockIdx.x * WG + threadIdx.x;
-    const int ns = min((int)*hcount, HCAP);
+    const int ns = hard_split((int)*hcount);
     if (t < ns * (NACC + 1)) hard_sum(hseg, t, hall[t / (NACC + 1)].x, cap, xh);
 }
 

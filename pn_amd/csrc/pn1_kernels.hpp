@@ -854,10 +854,14 @@ constexpr int HWG = 256;
 // at n = 131073) took ~460 us on one CU, on the critical path of every
 // mixed evaluation; 32 units on 32 CUs take ~1/32 of that.  The segments'
 // sums go to a scratch of HCAP stars x HSEG x (NACC + 1) and are added in
-// segment order (k_hard_sum); stars beyond the first HCAP of the list take
-// their row in one unit.
+// segment order (k_hard_sum).  An evaluation with more than HCAP hard
+// stars takes every row in one unit: whether a row is split depends only
+// on the count, never on a star's place in the list (which in an ensemble
+// follows the order the clusters' workgroups appended them).
 constexpr int HSEG = 32;
 constexpr int HCAP = 1024;
+
+__device__ __forceinline__ int hard_split(int nh) { return nh <= HCAP ? nh : 0; }
 
 __device__ __forceinline__ void hard_row(const JRec64* __restrict__ jr,
                                          const double* __restrict__ st, int cap, double eps2,
@@ -905,9 +909,10 @@ __device__ __forceinline__ void hard_row(const JRec64* __restrict__ jr,
     __syncthreads();
 }
 
-// Work unit u of the rows of nh hard stars of one cluster whose stars are
-// [p0, pend): the first min(nh, HCAP) stars in HSEG segments each (unit
-// h * HSEG + g: star h, segment g, into hseg), the others whole (into xh).
+// Work unit u of the rows of the hard stars, ns = hard_split(nh) of them
+// split (unit h * HSEG + g: star h, segment g, into hseg), for a star of a
+// cluster whose stars are [p0, pend); the units past ns * HSEG are whole
+// rows (into xh).
 __device__ __forceinline__ void hard_unit(const JRec64* __restrict__ jr,
                                           const double* __restrict__ st, int cap, double eps2,
                                           int u, int ns, int star, int p0, int pend,
@@ -922,8 +927,8 @@ __device__ __forceinline__ void hard_unit(const JRec64* __restrict__ jr,
     }
 }
 
-// The hard stars hlist[0 .. fn[1]): ns = min(fn[1], HCAP) of them in HSEG
-// units each, the rest one unit each; the workgroups take the units in turn.
+// The hard stars hlist[0 .. fn[1]): all in HSEG units each, or, beyond
+// HCAP of them, all whole; the workgroups take the units in turn.
 __global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr,
                                                   const double* __restrict__ st,
                                                   const int* __restrict__ hlist,
@@ -931,7 +936,7 @@ __global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr
                                                   double* __restrict__ xh,
                                                   double* __restrict__ hseg, Phys P)
 {
-    const int nh = fn[1], ns = min(nh, HCAP), nu = ns * HSEG + (nh - ns);
+    const int nh = fn[1], ns = hard_split(nh), nu = ns * HSEG + (nh - ns);
     for (int u = blockIdx.x; u < nu; u += gridDim.x) {
         const int h = u < ns * HSEG ? u / HSEG : ns + (u - ns * HSEG);
         hard_unit(jr, st, P.cap, P.eps2, u, ns, hlist[h], 1, P.n, xh, hseg);
@@ -956,7 +961,7 @@ __global__ __launch_bounds__(WG) void k_hard_sum(const int* __restrict__ hlist,
                                                 double* __restrict__ xh, int cap)
 {
     const int t = blockIdx.x * WG + threadIdx.x;
-    const int ns = min(fn[1], HCAP);
+    const int ns = hard_split(fn[1]);
     if (t < ns * (NACC + 1)) hard_sum(hseg, t, hlist[t / (NACC + 1)], cap, xh);
 }
 

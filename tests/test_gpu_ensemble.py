@@ -239,6 +239,33 @@ def test_hard_stars_of_many_clusters(ens, ctx, clusters):
         assert np.array_equal(a32[k], ctx.force()), k
 
 
+def test_more_hard_stars_than_split_rows():
+    """1100 lattice clusters of 27 stars, each with a centre star whose terms
+    cancel (sum m w^2 / |a| ~ 250): more hard stars than the 1024 whose rows
+    are split in segments, so every row is taken whole.  Mixed within 1e-6
+    of FP64 for every star, and repeated evaluations bitwise identical (the
+    split must not depend on the order in which the clusters' workgroups
+    list their hard stars)."""
+    lat = [lattice_cluster(1, 0.1, 1000 + k) for k in range(1100)]
+    x0, _, m0 = lat[0]
+    kr = cancel_ratio(x0, m0, ob.orc_force(*lat[0], EPS, C))
+    assert kr.max() > 3 * HARD_K
+    n = sum(len(m) for _, _, m in lat)
+    with pn.Ensemble(capacity=n, max_clusters=len(lat)) as e:
+        e.set_params(EPS, C, DT, pn.FP64)
+        e.upload(lat)
+        a64 = np.concatenate(e.force())
+        e.set_params(EPS, C, DT, pn.MIXED)
+        e.upload(lat)
+        e.force()
+        a32 = np.concatenate(e.force())
+        err = rel_err(a32, a64)
+        print(f"{len(lat)} lattice clusters: mixed vs FP64 max {err.max():.3e}", flush=True)
+        assert err.max() <= 1e-6
+        for _ in range(2):
+            assert np.array_equal(np.concatenate(e.force()), a32)
+
+
 def test_graph_replay_equals_eager_profiled(ens, clusters):
     """pn1_ens_step replays a captured graph; with profiling on it launches
     eagerly.  Both must give the same bits."""
