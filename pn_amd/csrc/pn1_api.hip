@@ -603,11 +603,15 @@ int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, in
     ++c->gen;
     c->n = n;
     c->m0 = m[0];
+    // mass scale of the mixed records: the largest |m_j| of the stars (0 if
+    // every star is massless: their records then have mass 0, and no pair
+    // needs a threshold); the equal-mass kernels take unit record masses,
+    // so only for stars of one positive mass
     c->mmax = 0.0;
-    c->eqm = true;
+    c->eqm = n > 1 && m[1] > 0.0;
     double dmax2 = 0.0;
     for (int i = 1; i < n; ++i) {
-        c->mmax = std::max(c->mmax, m[i]);
+        c->mmax = std::max(c->mmax, std::fabs(m[i]));
         c->eqm = c->eqm && m[i] == m[1];
         double d2 = 0.0;
         for (int k = 0; k < 3; ++k) d2 += (x[3 * i + k] - x[k]) * (x[3 * i + k] - x[k]);

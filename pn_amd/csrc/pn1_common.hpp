@@ -80,7 +80,7 @@ static_assert(sizeof(JRec32) == 64, "JRec32 layout");
 struct Phys {
     double eps, eps2, lam, dt, m0;
     double ws;       // 1/sqrt(eps^2): the reference's self-pair w
-    double mmax;     // largest star mass (mixed form's pair thresholds)
+    double mmax;     // largest |m| of the stars (mixed form; 0: all massless)
     float eps2f, m0f;
     int n, cap;      // particles incl. the BH; SoA stride
 };
@@ -139,7 +139,7 @@ struct MixPrep {
     Thr* thr;
     const double* aprev;    // previous evaluation (both null after an upload)
     const float* kprev;
-    double mmax, tauf;      // the cluster's largest star mass; BH-close threshold
+    double mmax, tauf;      // the cluster's largest |m| of its stars; BH-close threshold
     double lam;
     int hard_rows;          // stars beyond PN1_HARD_K get FP64 rows of their own
 };

@@ -151,7 +151,7 @@ __device__ __forceinline__ bool prep_star(const double* __restrict__ st,
         h.yl = (float)(y - (double)h.yh);
         h.zl = (float)(z - (double)h.zh);
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
-        h.m = (float)(mp / M.mmax);
+        h.m = M.mmax > 0.0 ? (float)(mp / M.mmax) : 0.0f;     // no scale: massless stars
         h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
         h.Qdx = (float)r.Qdx; h.Qdy = (float)r.Qdy; h.Qdz = (float)r.Qdz;
     }

@@ -135,6 +135,49 @@ def test_tapered_split_small_n(ctx, n_stars):
         ctx.set_tuning(0, 0, -1)
 
 
+@pytest.mark.parametrize("massless", ["all", "half"])
+def test_massless_stars(ctx, massless):
+    """Test particles: stars of mass 0 feel the BH and the massive stars but
+    pull on nothing.  With every star massless the mixed records have no
+    mass scale (they carry mass 0, and no pair needs a close-pair or
+    lo-part threshold); with half of them massless the equal-mass kernels
+    must not be used.  FP64 against
+    the oracle to 1e-12, mixed within 1e-6 of FP64 (stars and BH), in a
+    context and in an ensemble beside an ordinary cluster."""
+    x, v, m = pn.ic_plummer(300, 71, a=1.0, m_bh=0.05)
+    m = m.copy()
+    if massless == "all":
+        m[1:] = 0.0
+    else:
+        m[1::2] = 0.0
+    ref = ob.orc_force(x, v, m, 1e-3, 100.0)
+    ctx.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+    ctx.upload(x, v, m)
+    a64 = ctx.force()
+    err = rel_err(a64, ref)
+    assert err.max() <= 1e-12, err.max()
+    ctx.set_params(1e-3, 100.0, 2.0**-10, pn.MIXED)
+    ctx.upload(x, v, m)
+    ctx.force()
+    a32 = ctx.force()
+    e32 = rel_err(a32, a64)
+    print(f"{massless} massless: FP64 vs oracle {err.max():.3e}, mixed vs FP64 {e32.max():.3e}",
+          flush=True)
+    assert np.isfinite(a32).all() and e32.max() <= 1e-6
+    other = pn.ic_plummer(400, 72, a=1.0, m_bh=0.02)
+    with pn.Ensemble(capacity=800, max_clusters=2) as e:
+        e.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+        e.upload([(x, v, m), other])
+        b64 = e.force()
+        e.set_params(1e-3, 100.0, 2.0**-10, pn.MIXED)
+        e.upload([(x, v, m), other])
+        e.force()
+        b32 = e.force()
+    assert rel_err(b64[0], ref).max() <= 1e-12
+    for k in range(2):
+        assert np.isfinite(b32[k]).all() and rel_err(b32[k], b64[k]).max() <= 1e-6, k
+
+
 @pytest.mark.parametrize("name", ["traj_rand_n65_c1", "traj_c1_n1025_c100"])
 def test_fp64_trajectory(ctx, name):
     d = load_fixture(name)
