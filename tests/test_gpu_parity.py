@@ -142,8 +142,9 @@ def test_massless_stars(ctx, massless):
     mass scale (they carry mass 0, and no pair needs a close-pair or
     lo-part threshold); with half of them massless the equal-mass kernels
     must not be used.  FP64 against
-    the oracle to 1e-12, mixed within 1e-6 of FP64 (stars and BH), in a
-    context and in an ensemble beside an ordinary cluster."""
+    the oracle to 1e-12 (and the energy terms within the oracle's rounding
+    bound), mixed within 1e-6 of FP64 (stars and BH), in a context and in an
+    ensemble beside an ordinary cluster."""
     x, v, m = pn.ic_plummer(300, 71, a=1.0, m_bh=0.05)
     m = m.copy()
     if massless == "all":
@@ -156,6 +157,9 @@ def test_massless_stars(ctx, massless):
     a64 = ctx.force()
     err = rel_err(a64, ref)
     assert err.max() <= 1e-12, err.max()
+    eref = ob.orc_energy_terms(x, v, m, 1e-3, 100.0)
+    en = ctx.energy()
+    assert np.abs(en - eref).max() <= energy_atol(dict(m=m, eps=1e-3), eref), (en, eref)
     ctx.set_params(1e-3, 100.0, 2.0**-10, pn.MIXED)
     ctx.upload(x, v, m)
     ctx.force()
@@ -176,6 +180,29 @@ def test_massless_stars(ctx, massless):
     assert rel_err(b64[0], ref).max() <= 1e-12
     for k in range(2):
         assert np.isfinite(b32[k]).all() and rel_err(b32[k], b64[k]).max() <= 1e-6, k
+
+
+def test_coincident_stars(ctx):
+    """Stars at one point (the reference's softening keeps their pair finite:
+    w = 1/eps, r = 0) and a pair 1e-7 apart, well inside eps: FP64 against
+    the oracle to 1e-12, mixed within 1e-6 of FP64."""
+    x, v, m = pn.ic_plummer(256, 73, a=1.0, m_bh=0.05)
+    x = x.copy()
+    x[5] = x[4]
+    x[6] = x[4]
+    x[8] = x[7] + [1e-7, 0.0, 0.0]
+    ref = ob.orc_force(x, v, m, 1e-3, 100.0)
+    ctx.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+    ctx.upload(x, v, m)
+    a64 = ctx.force()
+    ctx.set_params(1e-3, 100.0, 2.0**-10, pn.MIXED)
+    ctx.upload(x, v, m)
+    ctx.force()
+    a32 = ctx.force()
+    err, e32 = rel_err(a64, ref), rel_err(a32, a64)
+    print(f"coincident stars: FP64 vs oracle {err.max():.3e}, mixed vs FP64 {e32.max():.3e}",
+          flush=True)
+    assert err.max() <= 1e-12 and e32.max() <= 1e-6
 
 
 @pytest.mark.parametrize("name", ["traj_rand_n65_c1", "traj_c1_n1025_c100"])
