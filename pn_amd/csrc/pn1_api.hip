@@ -599,7 +599,9 @@ int pn1_upload(pn1_ctx* c, const double* x, const double* v, const double* m, in
                                  hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipMemcpyAsync(c->m, h + 6 * cap, n * sizeof(double), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    if (n != c->n || m[0] != c->m0) drop_graph(c);
+    // a captured step holds this state's launch constants (n, the BH mass,
+    // the mixed form's mass scale, equal-mass kernels, BH-close threshold)
+    drop_graph(c);
     ++c->gen;
     c->n = n;
     c->m0 = m[0];

@@ -60,6 +60,37 @@ def test_graph_replay_equals_eager(big):
     assert np.array_equal(xe, xg) and np.array_equal(ve, vg)
 
 
+@pytest.mark.parametrize("mode", [pn.FP64, pn.MIXED])
+def test_upload_after_captured_steps(mode):
+    """A captured step holds its state's launch constants (the mixed form's
+    mass scale and equal-mass kernels, its BH-close threshold).  Steps after
+    uploading another state of the same n and BH mass (equal star masses,
+    then unequal ones of a larger scale, then a wider cluster) equal those
+    of a fresh context, bit for bit."""
+    x, v, m = pn.ic_plummer(3000, 81, a=1.0, m_bh=0.05)
+    rng = np.random.default_rng(82)
+    m2 = m.copy()
+    m2[1:] *= rng.uniform(0.5, 3.0, len(m) - 1)
+    x3 = x.copy()
+    x3[1:] = x[0] + 4.0 * (x[1:] - x[0])
+    states = [(x, v, m), (x, v, m2), (x3, v, m2)]
+    eps, c, dt = 1e-3, 50.0, 2.0**-10
+    with pn.Context(len(m)) as ctx:
+        ctx.set_params(eps, c, dt, mode)
+        got = []
+        for st in states:
+            ctx.upload(*st)
+            ctx.step(2)
+            got.append(ctx.download())
+    for st, (xg, vg) in zip(states, got):
+        with pn.Context(len(m)) as fresh:
+            fresh.set_params(eps, c, dt, mode)
+            fresh.upload(*st)
+            fresh.step(2)
+            xf, vf = fresh.download()
+        assert np.array_equal(xg, xf) and np.array_equal(vg, vf)
+
+
 def test_tuning_invariance(big):
     ctx, x, v, m = big
     ref = force_at(ctx, x, v, m, pn.FP64, C3["eps"], C3["c"], tuning=(64, 0, 0))
