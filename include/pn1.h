@@ -136,7 +136,9 @@ int  pn1_force_noalloc(const double *st, const double *m, int n, double eps,
 int  pn1_energy(pn1_ctx *ctx, pn1_energy_t *out);
 /* `steps` steps of the reference's three-evaluation stage scheme
  * (Limit-sticky9.c:841-893), state kept on the device; the step is
- * captured once into a hipGraph and replayed. */
+ * captured into a hipGraph on the first call and replayed after that,
+ * until pn1_upload, pn1_set_params or pn1_set_tuning changes what it
+ * holds (the next call captures it again). */
 int  pn1_step(pn1_ctx *ctx, int steps);
 /* Blocks until the context's stream is idle. */
 int  pn1_sync(pn1_ctx *ctx);
