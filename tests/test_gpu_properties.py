@@ -327,12 +327,13 @@ def test_ens_autotune_keeps_results():
 def test_gpu_matches_oracle_on_random_small_systems():
     """hypothesis draws small systems (n = 1 ... 40, random masses, eps, c,
     BH off-centre and moving): FP64 forces and energies of the HIP path
-    against the oracle."""
+    against the oracle.  Derandomized: every run checks the same 40 draws,
+    so a run on a fresh machine checks what the previous runs checked."""
     from hypothesis import given, settings, strategies as st
 
     ctx = pn.Context(capacity=64)
 
-    @settings(max_examples=40, deadline=None)
+    @settings(max_examples=40, deadline=None, derandomize=True, database=None)
     @given(st.integers(1, 40), st.integers(0, 2**31 - 1), st.sampled_from([1e-3, 1e-2, 0.1]),
            st.sampled_from([1.0, 3.0, 100.0]))
     def check(n, seed, eps, c):

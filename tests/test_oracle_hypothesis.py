@@ -25,7 +25,9 @@ def systems(draw):
     return x, v, m, eps, c, rng
 
 
-SET = settings(max_examples=40, deadline=None)
+# the same 40 draws on every run (no example database): a fresh machine
+# checks what earlier runs checked
+SET = settings(max_examples=40, deadline=None, derandomize=True, database=None)
 
 
 def rotation(rng):
