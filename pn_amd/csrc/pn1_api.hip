@@ -535,6 +535,9 @@ int pn1_set_params(pn1_ctx* c, double eps, double cl, double dt, int mode)
     if (mode != PN1_FP64 && mode != PN1_MIXED) return seterr(c, PN1_EINVAL, "unknown mode %d", mode);
     if (eps != c->eps || cl != c->c || dt != c->dt || mode != c->mode) drop_graph(c);
     if (eps != c->eps || cl != c->c || dt != c->dt) ++c->gen;
+    // the mixed form's pair thresholds come from the last evaluation's
+    // accelerations, which depend on eps and c: take them afresh
+    if (eps != c->eps || cl != c->c) c->acc_ok = false;
     c->eps = eps;
     c->c = cl;
     c->dt = dt;

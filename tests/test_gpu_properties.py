@@ -91,6 +91,30 @@ def test_upload_after_captured_steps(mode):
         assert np.array_equal(xg, xf) and np.array_equal(vg, vf)
 
 
+def test_params_change_renews_mixed_thresholds():
+    """The mixed form's pair thresholds come from the previous evaluation's
+    accelerations; after eps or c changes they are taken afresh, so the
+    forces equal a fresh context's bit for bit (context and ensemble)."""
+    x, v, m = pn.ic_plummer(3000, 83, a=1.0, m_bh=0.05)
+    first, second = (1e-4, 50.0), (3e-2, 100.0)
+    with pn.Context(len(m)) as ctx, pn.Context(len(m)) as fresh:
+        ctx.set_params(*first, 2.0**-10, pn.MIXED)
+        ctx.upload(x, v, m)
+        ctx.force()
+        ctx.set_params(*second, 2.0**-10, pn.MIXED)
+        fresh.set_params(*second, 2.0**-10, pn.MIXED)
+        fresh.upload(x, v, m)
+        assert np.array_equal(ctx.force(), fresh.force())
+    with pn.Ensemble(len(m), 1) as ens, pn.Ensemble(len(m), 1) as fresh:
+        ens.set_params(*first, 2.0**-10, pn.MIXED)
+        ens.upload([(x, v, m)])
+        ens.force()
+        ens.set_params(*second, 2.0**-10, pn.MIXED)
+        fresh.set_params(*second, 2.0**-10, pn.MIXED)
+        fresh.upload([(x, v, m)])
+        assert np.array_equal(ens.force()[0], fresh.force()[0])
+
+
 def test_tuning_invariance(big):
     ctx, x, v, m = big
     ref = force_at(ctx, x, v, m, pn.FP64, C3["eps"], C3["c"], tuning=(64, 0, 0))
