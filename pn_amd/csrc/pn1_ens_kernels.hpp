@@ -102,7 +102,7 @@ __device__ __forceinline__ void ens_pair32(const JRec32* __restrict__ jr,
         if (it < 0) break;
         const EItem w = items[it];
         const EGeom g = ens_geom(w);
-        const I32 I = load_i32(j64, thr, st, P.cap, g.i, P.eps2);
+        const I32 I = load_i32(j64, thr, st, P.cap, g.i, w.off, P.eps2);
         if (!g.live) continue;                 // as in k_ens_pair64
         float a[NACC32], sN[3], cN[3];
 #pragma unroll

@@ -63,9 +63,9 @@ __device__ __forceinline__ void stage_update(int p, int cap, const double (&a)[3
 // ---------------------------------------------------------------------
 // k_prep: per-star BH geometry + separable sums
 // ---------------------------------------------------------------------
-// Mixed form's pair thresholds of a star at x whose previous evaluation
-// gave the acceleration aprev and cancellation ratio kprev (null: none
-// known), for star masses up to mmax (see Thr).
+// Mixed form's pair thresholds of a star at x from its BH whose previous
+// evaluation gave the acceleration aprev and cancellation ratio kprev
+// (null: none known), for star masses up to mmax (see Thr).
 __device__ __forceinline__ Thr pair_thresholds(double x, double y, double z,
                                                const double* aprev, const float* kprev,
                                                size_t cap, int p, double mmax, bool rows,
@@ -146,17 +146,19 @@ __device__ __forceinline__ bool prep_star(const double* __restrict__ st,
         h.A = h.B = h.MB = 0.0f;
         h.Qdx = h.Qdy = h.Qdz = 0.0f;
     } else {
-        h.xh = (float)x; h.yh = (float)y; h.zh = (float)z;
-        h.xl = (float)(x - (double)h.xh);
-        h.yl = (float)(y - (double)h.yh);
-        h.zl = (float)(z - (double)h.zh);
+        // double-single position relative to the BH: the FP32 hi parts then
+        // round at the cluster's own scale, wherever the cluster sits
+        h.xh = (float)dx; h.yh = (float)dy; h.zh = (float)dz;
+        h.xl = (float)(dx - (double)h.xh);
+        h.yl = (float)(dy - (double)h.yh);
+        h.zl = (float)(dz - (double)h.zh);
         h.sx = (float)r.sx; h.sy = (float)r.sy; h.sz = (float)r.sz;
         h.m = M.mmax > 0.0 ? (float)(mp / M.mmax) : 0.0f;     // no scale: massless stars
         h.A = (float)r.A; h.B = (float)r.B; h.MB = (float)r.MB;
         h.Qdx = (float)r.Qdx; h.Qdy = (float)r.Qdy; h.Qdz = (float)r.Qdz;
     }
     M.j32[p] = h;
-    M.thr[p] = pair_thresholds(x, y, z, M.aprev, M.kprev, (size_t)cap, p, M.mmax,
+    M.thr[p] = pair_thresholds(dx, dy, dz, M.aprev, M.kprev, (size_t)cap, p, M.mmax,
                                M.hard_rows != 0, hard);
     return flagged;
 }
@@ -411,23 +413,31 @@ struct I32 {
     float xh, yh, zh, xl, yl, zl, gx, gy, gz, hx, hy, hz, D2e, rc2, dn2;
 };
 
-// Star i: its double-single position, velocity, |d_i|^2 (from its FP64
-// record: its mixed record is neutral if the star is close to the BH) and
-// pair thresholds.
+// Star i: its double-single position relative to its BH (particle b0; as
+// in its mixed record, which is neutral if the star is close to the BH, so
+// it is rebuilt from the FP64 record), velocity, |d_i|^2 and pair
+// thresholds.
 __device__ __forceinline__ I32 load_i32(const JRec64* __restrict__ j64,
                                         const Thr* __restrict__ thr,
-                                        const double* __restrict__ st, int cap, int i,
+                                        const double* __restrict__ st, int cap, int i, int b0,
                                         double eps2)
 {
     const JRec64& r = j64[i];
     const double vx = st[3 * cap + i], vy = st[4 * cap + i], vz = st[5 * cap + i];
     const double D2 = r.D2;
     const Thr t = thr[i];
+    double dx, dy, dz;
+    {
+#pragma clang fp contract(off)
+        dx = r.x - st[b0];
+        dy = r.y - st[cap + b0];
+        dz = r.z - st[2 * cap + b0];
+    }
     I32 I;
-    I.xh = (float)r.x; I.yh = (float)r.y; I.zh = (float)r.z;
-    I.xl = (float)(r.x - (double)I.xh);
-    I.yl = (float)(r.y - (double)I.yh);
-    I.zl = (float)(r.z - (double)I.zh);
+    I.xh = (float)dx; I.yh = (float)dy; I.zh = (float)dz;
+    I.xl = (float)(dx - (double)I.xh);
+    I.yl = (float)(dy - (double)I.yh);
+    I.zl = (float)(dz - (double)I.zh);
     I.gx = (float)(4.0 * vx); I.gy = (float)(4.0 * vy); I.gz = (float)(4.0 * vz);
     I.hx = (float)(PN1_4OSQ15 * vx); I.hy = (float)(PN1_4OSQ15 * vy);
     I.hz = (float)(PN1_4OSQ15 * vz);
@@ -466,8 +476,9 @@ __device__ __forceinline__ void two_sum_into(float s, float y, float& u, float& 
 // Close pairs (R2e < rc2, not the self pair), taken when any lane of the
 // wave has one; `close` selects the lanes.  A close pair's Newtonian term
 // (m_j / mscale) w^3 r is evaluated in FP64 (rscale = 1 / mscale), from
-// j's FP64 record and the lane's double-single position, and goes into the
-// Kahan total; the FP32 term
+// the mass of j's FP64 record and the double-single positions of j (its
+// mixed record Js) and of the lane, and goes into the Kahan total; the
+// FP32 term
 // mw3 r that the caller adds next is cancelled exactly: the short-run sum
 // is folded, then set to -p with p = fl(mw3 r), so the caller's
 // fma(mw3, r, -p) leaves the exact product residual e = mw3 r - p, and
@@ -475,21 +486,23 @@ __device__ __forceinline__ void two_sum_into(float s, float y, float& u, float& 
 // compensation).  Every lane computes; the other lanes keep their sums.
 // The branch around it is wave-uniform, so j and the record stream stay
 // scalar.
-__device__ __forceinline__ void close_pairs(bool close, const JRec64& J, float jm32,
+__device__ __forceinline__ void close_pairs(bool close, const JRec64& J, const JRec32& Js,
                                             const I32& I, double eps2, double rscale, float mw3,
                                             float rx, float ry, float rz, float (&a)[NACC32],
                                             float (&sN)[3], float (&cN)[3])
 {
 #pragma clang fp contract(off)
-    // the lane's position (double-single: hi + lo is x to 2^-48), made
-    // opaque here so that its FP64 form is built in this rarely taken branch
-    // rather than hoisted out of the j loop and kept live across it
+    // the lane's position (double-single, relative to the BH: hi + lo is it
+    // to 2^-48), made opaque here so that its FP64 form is built in this
+    // rarely taken branch rather than hoisted out of the j loop and kept
+    // live across it; j's from its mixed record, the same way
     float ph[3] = {I.xh, I.yh, I.zh}, pl[3] = {I.xl, I.yl, I.zl};
 #pragma unroll
     for (int k = 0; k < 3; ++k) __asm__ volatile("" : "+v"(ph[k]), "+v"(pl[k]));
-    const double dx = J.x - ((double)ph[0] + (double)pl[0]);
-    const double dy = J.y - ((double)ph[1] + (double)pl[1]);
-    const double dz = J.z - ((double)ph[2] + (double)pl[2]);
+    const double dx = ((double)Js.xh + (double)Js.xl) - ((double)ph[0] + (double)pl[0]);
+    const double dy = ((double)Js.yh + (double)Js.yl) - ((double)ph[1] + (double)pl[1]);
+    const double dz = ((double)Js.zh + (double)Js.zl) - ((double)ph[2] + (double)pl[2]);
+    const float jm32 = Js.m;
     const double R2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
     const double w = rsq64(R2);
     // in the record's mass units; none for a star taken out of the FP32 pass
@@ -566,7 +579,7 @@ __device__ __forceinline__ void body32(const JRec32& J, const JRec64* __restrict
         if (__builtin_amdgcn_ballot_w64(close)) {
             const float wc = rsq32<RF>(R2e);
             const float mw3c = (EQ && !MASK ? wc : mj * wc) * (wc * wc);
-            close_pairs(close, j64[j], J.m, I, eps2d, rscale, mw3c, rx, ry, rz, a, sN, cN);
+            close_pairs(close, j64[j], J, I, eps2d, rscale, mw3c, rx, ry, rz, a, sN, cN);
         }
     }
     const float w = rsq32<RF>(R2e);
@@ -684,7 +697,7 @@ __global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
     const int n = P.n;
     const int li = ib * WG + threadIdx.x + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(j64, thr, st, P.cap, i, P.eps2);
+    const I32 I = load_i32(j64, thr, st, P.cap, i, 0, P.eps2);
     const JRange R = j_range(s, S, n, ib);
     const float eps2 = P.eps2f;
     const double eps2d = P.eps2, rscale = 1.0 / P.mmax;
@@ -742,7 +755,7 @@ void k_pair32q(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64,
     const int n = P.n;
     const int li = ib * WG + iw * 64 + lane + 1;
     const int i = li < n ? li : n - 1;
-    const I32 I = load_i32(j64, thr, st, P.cap, i, P.eps2);
+    const I32 I = load_i32(j64, thr, st, P.cap, i, 0, P.eps2);
     // chunk 4 s4 + sub of the S-chunk split (k_pair32's bounds), and the
     // self window of this wave's 64 stars
     JRange R = j_range(s4 * QSUB + sub, S, n, ib);

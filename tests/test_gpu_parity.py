@@ -205,6 +205,35 @@ def test_coincident_stars(ctx):
     assert err.max() <= 1e-12 and e32.max() <= 1e-6
 
 
+@pytest.mark.parametrize("shift", [(1000.0, -500.0, 250.0), (1e5, 0.0, 0.0)])
+def test_cluster_far_from_origin(ctx, shift):
+    """A cluster whose BH sits far from the origin: FP32 hi parts of the
+    positions keep only ~2^-24 |x| (6e-5 at |x| = 1000, 6e-3 at 1e5), and the
+    double-single separation must carry the rest.  FP64 against the oracle
+    to 1e-12; mixed within 1e-6 of FP64, at the initial state and after two
+    FP64 steps (positions with all 53 bits)."""
+    x, v, m = pn.ic_plummer(2000, 74, a=1.0, m_bh=0.05)
+    x = x + np.array(shift)
+    for steps in (0, 2):
+        ctx.set_params(1e-3, 100.0, 2.0**-10, pn.FP64)
+        ctx.upload(x, v, m)
+        if steps:
+            ctx.step(steps)
+            xs, vs = ctx.download()
+            ctx.upload(xs, vs, m)
+        else:
+            xs, vs = x, v
+        a64 = ctx.force()
+        err = rel_err(a64, ob.orc_force(xs, vs, m, 1e-3, 100.0))
+        ctx.set_params(1e-3, 100.0, 2.0**-10, pn.MIXED)
+        ctx.upload(xs, vs, m)
+        ctx.force()
+        e32 = rel_err(ctx.force(), a64)
+        print(f"shift {shift} +{steps}: FP64 vs oracle {err.max():.3e}, mixed vs FP64 "
+              f"{e32.max():.3e}", flush=True)
+        assert err.max() <= 1e-12 and e32.max() <= 1e-6
+
+
 @pytest.mark.parametrize("name", ["traj_rand_n65_c1", "traj_c1_n1025_c100"])
 def test_fp64_trajectory(ctx, name):
     d = load_fixture(name)
