@@ -124,9 +124,10 @@ def roofline(n, avg_ms, peak, traffic=None, valu=None):
 
 
 def sources_sha256():
-    """Hash of the library's sources (pn_amd/csrc/*), which every committed
-    counter summary records: counters measured on other sources are not
-    reported as this build's."""
+    """Hash of the library's sources (pn_amd/csrc/*) and of its build recipe
+    (pn_amd/build.py: compiler flags), which every committed counter summary
+    records: counters measured on another build are not reported as this
+    build's."""
     import hashlib
     h = hashlib.sha256()
     d = os.path.join(ROOT, "pn_amd", "csrc")
@@ -134,6 +135,8 @@ def sources_sha256():
         if name.endswith((".hip", ".hpp", ".inc", ".h", ".c")):
             with open(os.path.join(d, name), "rb") as f:
                 h.update(name.encode() + b"\0" + f.read())
+    with open(os.path.join(ROOT, "pn_amd", "build.py"), "rb") as f:
+        h.update(b"build.py\0" + f.read())
     return h.hexdigest()
 
 

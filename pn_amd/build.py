@@ -54,13 +54,18 @@ def build(verbose=False):
     _run(["gcc", *C_FLAGS, *inc, "-c", os.path.join(CSRC, "pn1_host.c"), "-o", host_o], verbose)
     adj_o = os.path.join(BUILD, "pn1_adj.o")
     _run([hipcc, *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "pn1_api.hip"), "-o", api_o], verbose)
+    # the mixed-form pair kernels: LLVM's max-ILP machine scheduler (see
+    # pn1_pair32.hip)
+    p32_o = os.path.join(BUILD, "pn1_pair32.o")
+    _run([hipcc, *HIP_FLAGS, "-mllvm", "-amdgpu-sched-strategy=max-ilp", *inc, "-c",
+          os.path.join(CSRC, "pn1_pair32.hip"), "-o", p32_o], verbose)
     # the adjoint's dual-number kernels: finite values, no signed zeros, so
     # products with zero seed tangents fold away (see pn1_adj.hip)
     _run([hipcc, *HIP_FLAGS, "-ffinite-math-only", "-fno-signed-zeros", *inc, "-c",
           os.path.join(CSRC, "pn1_adj.hip"), "-o", adj_o], verbose)
     tmp = LIB + ".tmp"
-    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, adj_o, host_o, "-o", tmp],
-         verbose)
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", api_o, p32_o, adj_o, host_o, "-o",
+          tmp], verbose)
     os.replace(tmp, LIB)
     sticky = os.path.join(CSRC, "sticky9.c")
     if os.path.exists(sticky):

@@ -1,6 +1,7 @@
 // pn1_api.hip -- host side of libpn1: the context API, the three legacy
 // shapes of Limit-sticky9 (force, energy, initial_energy), the autotuner and,
-// through pn1_ens.inc, the ensemble API.  The adjoint is in pn1_adj.hip.
+// through pn1_ens.inc, the ensemble API.  The adjoint is in pn1_adj.hip, the
+// launches of the mixed-form pair kernels in pn1_pair32.hip.
 //
 // Device layout (FP64 SoA, stride cap, particle 0 = BH):
 //   X[6][cap]   state at the start of a step (x, y, z, vx, vy, vz)
@@ -31,6 +32,7 @@
 #include "pn1_ens.hpp"
 #include "pn1_ens_kernels.hpp"
 #include "pn1_kernels.hpp"
+#include "pn1_pair32.hpp"
 
 using namespace pn1dev;
 
@@ -218,28 +220,6 @@ void launch_fin(pn1_ctx* c, const double* in, const Phys& P)
                        c->Y, c->H1, c->H2, c->acc, P);
 }
 
-template <int PP, bool TS, bool EQ>
-void launch32(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, int S)
-{
-    if (quad_pairs(c, S))
-        hipLaunchKernelGGL((k_pair32q<PP, TS, EQ>), dim3(pair_grid(S / QSUB, nib)), dim3(WGQ), 0,
-                           c->st, c->j32, c->j64, c->thr, in, (float*)c->part, P, S, nib, ib0,
-                           c->tile);
-    else
-        hipLaunchKernelGGL((k_pair32<PP, TS, EQ>), dim3(pair_grid(S, nib)), dim3(WG), 0, c->st,
-                           c->j32, c->j64, c->thr, in, (float*)c->part, P, S, nib, ib0, c->tile);
-}
-
-// mixed pair kernels by [variant][cancellation sums][equal masses]
-using Launch32 = void (*)(pn1_ctx*, const double*, const Phys&, int, int, int);
-const Launch32 k_launch32[16] = {
-    launch32<0, false, false>, launch32<0, false, true>, launch32<0, true, false>,
-    launch32<0, true, true>,   launch32<1, false, false>, launch32<1, false, true>,
-    launch32<1, true, false>,  launch32<1, true, true>,   launch32<2, false, false>,
-    launch32<2, false, true>,  launch32<2, true, false>,  launch32<2, true, true>,
-    launch32<3, false, false>, launch32<3, false, true>,  launch32<3, true, false>,
-    launch32<3, true, true>};
-
 // The pair kernel over i blocks ib0 .. ib0 + nib - 1 against every star,
 // with the j range split in S chunks.  ts (mixed form): also accumulate the
 // sums behind the stars' cancellation ratios.
@@ -249,7 +229,7 @@ void launch_pair(pn1_ctx* c, const double* in, const Phys& P, int ib0, int nib, 
     const dim3 g(pair_grid(S, nib)), blk(WG);
     const int variant = kernel_variant(c);
     if (mixed) {
-        k_launch32[(variant & 3) * 4 + (ts ? 2 : 0) + (c->eqm ? 1 : 0)](c, in, P, ib0, nib, S);
+        launch_pair32(c, in, P, ib0, nib, S, pair32_form(variant, ts, c->eqm));
     } else if (variant & 1) {
         hipLaunchKernelGGL(k_pair64<1>, g, blk, 0, c->st, c->j64, in, (double*)c->part, P, S, nib,
                            ib0);

@@ -22,6 +22,9 @@
 // and so is the order of each lane's j loop: with one chunk per cluster
 // the results equal a single-cluster run with an unsplit j range bit for
 // bit.
+//
+// Included by pn1_api.hip and pn1_pair32.hip (the mixed-form pair kernels):
+// the kernels that are not templates are declared inline.
 #pragma once
 #include "pn1_ens_types.hpp"
 #include "pn1_kernels.hpp"
@@ -181,7 +184,7 @@ void k_ens_pair32x(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64
 // M: the mixed form's inputs (M.j32 null: FP64), with each cluster's largest
 // star mass cmmax[c] and BH-close threshold ctauf[c] in place of M.mmax,
 // M.tauf; fl, fblk: as k_prep.
-__global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
+inline __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
                                                 const double* __restrict__ m,
                                                 const double* __restrict__ cm0,
                                                 const int* __restrict__ coff,
@@ -226,7 +229,7 @@ __global__ __launch_bounds__(WG) void k_ens_prep(const double* __restrict__ st,
 // hall[0 .. *hcount) (k_ens_gsum zeroes *hcount).  The clusters' order in
 // hall depends on which workgroup reserves its range first; each hard row
 // is computed on its own, so the results do not.
-__global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
+inline __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
                                                  const int* __restrict__ fblk,
                                                  const int* __restrict__ cblk,
                                                  const int* __restrict__ coff,
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(WG) void k_ens_flist(const int* __restrict__ fl,
 // beyond HCAP entries, every one whole; the workgroups take the units in
 // turn.  With at most HCAP hard stars in the ensemble, each row is summed
 // as a single-cluster run sums it.
-__global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
+inline __global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict__ jr,
                                                       const double* __restrict__ st,
                                                       const int* __restrict__ coff,
                                                       const int2* __restrict__ hall,
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(HWG) void k_ens_hard_rows(const JRec64* __restrict_
 }
 
 // The split rows' segment sums, in segment order (as k_hard_sum).
-__global__ __launch_bounds__(WG) void k_ens_hard_sum(const int2* __restrict__ hall,
+inline __global__ __launch_bounds__(WG) void k_ens_hard_sum(const int2* __restrict__ hall,
                                                     const unsigned int* __restrict__ hcount,
                                                     const double* __restrict__ hseg,
                                                     double* __restrict__ xh, int cap)
@@ -281,7 +284,7 @@ __global__ __launch_bounds__(WG) void k_ens_hard_sum(const int2* __restrict__ ha
 
 // Per star block: its stars' FP64 pairs with their cluster's flagged stars
 // (k_pair_fl per cluster), into xp [NACC + 1][cap] at the star's index.
-__global__ __launch_bounds__(WG) void k_ens_pair_fl(const JRec64* __restrict__ jr,
+inline __global__ __launch_bounds__(WG) void k_ens_pair_fl(const JRec64* __restrict__ jr,
                                                    const double* __restrict__ st,
                                                    const int* __restrict__ coff,
                                                    const EBlk* __restrict__ blk,
@@ -311,7 +314,7 @@ __device__ __forceinline__ void cluster_rows_sum(const double* __restrict__ rows
         for (int k = 0; k < N; ++k) g[k] += rows[(size_t)b * N + k];
 }
 
-__global__ __launch_bounds__(WG) void k_ens_gsum(const double* __restrict__ gblk,
+inline __global__ __launch_bounds__(WG) void k_ens_gsum(const double* __restrict__ gblk,
                                                 const int* __restrict__ cblk,
                                                 double* __restrict__ gs,
                                                 unsigned int* __restrict__ counter)
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(WG) void k_ens_bh(const double* __restrict__ bhblk,
 // ---------------------------------------------------------------------
 // ensemble energies (FP64)
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ jr,
+inline __global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ jr,
                                                    const double* __restrict__ st,
                                                    double* __restrict__ part,
                                                    const EItem* __restrict__ items,
@@ -425,7 +428,7 @@ __global__ __launch_bounds__(WG) void k_ens_epair64(const JRec64* __restrict__ j
     }
 }
 
-__global__ __launch_bounds__(WG) void k_ens_efin(const double* __restrict__ st,
+inline __global__ __launch_bounds__(WG) void k_ens_efin(const double* __restrict__ st,
                                                 const double* __restrict__ m,
                                                 const double* __restrict__ cm0,
                                                 const int* __restrict__ coff,
@@ -453,7 +456,7 @@ __global__ __launch_bounds__(WG) void k_ens_efin(const double* __restrict__ st,
 }
 
 // out[c] = {kin, pot, pot_pn/c^2, pot_pn2/c^2, total} of cluster c
-__global__ __launch_bounds__(WG) void k_ens_ered(const double* __restrict__ eblk,
+inline __global__ __launch_bounds__(WG) void k_ens_ered(const double* __restrict__ eblk,
                                                 const int* __restrict__ cblk,
                                                 const int* __restrict__ coff,
                                                 const double* __restrict__ st,

@@ -22,6 +22,10 @@
 //
 // All reductions use a fixed order, so results are bitwise reproducible
 // for a given launch shape.
+//
+// Two translation units include this header: pn1_api.hip launches every
+// kernel but the mixed-form pair kernels, pn1_pair32.hip those.  The kernels
+// that are not templates are therefore declared inline.
 #pragma once
 #include "pn1_common.hpp"
 
@@ -166,7 +170,7 @@ __device__ __forceinline__ bool prep_star(const double* __restrict__ st,
 // Mixed form: fl per star, bit 0 whether it is taken out of the FP32 pass
 // (BH-close), bit 1 whether it gets an FP64 row of its own (hard); fblk per
 // block the number of each, [2 b] and [2 b + 1].  The BH mass is m[0].
-__global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
+inline __global__ __launch_bounds__(WG) void k_prep(const double* __restrict__ st,
                                             const double* __restrict__ m, Phys P,
                                             JRec64* __restrict__ j64,
                                             double* __restrict__ gblk, MixPrep M,
@@ -252,7 +256,7 @@ __device__ __forceinline__ int list_flagged(const int* __restrict__ fl,
 
 // The BH-close stars (flist, fn[0]) and the hard stars (hlist, fn[1]) of
 // k_prep's flags.
-__global__ __launch_bounds__(WG) void k_flist(const int* __restrict__ fl,
+inline __global__ __launch_bounds__(WG) void k_flist(const int* __restrict__ fl,
                                              const int* __restrict__ fblk, int nblk, int n,
                                              int* __restrict__ flist, int* __restrict__ hlist,
                                              int* __restrict__ fn)
@@ -682,13 +686,13 @@ __device__ __forceinline__ void run32(const JRec32* __restrict__ jr,
 // than PN1_CLOSE_KAPPA |a|.  The other sums enter scaled by 1/c^2 (or, for the
 // BH, beside its FP64 Newtonian term) and accumulate in plain FP32 over
 // the whole chunk.
+// The one-record form is held to 8 waves per SIMD: under the max-ILP
+// scheduler (pn1_pair32.hip) it would otherwise take 65 VGPRs.
 template <int PP, bool TS, bool EQ>
-__global__ __launch_bounds__(WG) void k_pair32(const JRec32* __restrict__ jr,
-                                              const JRec64* __restrict__ j64,
-                                              const Thr* __restrict__ thr,
-                                              const double* __restrict__ st,
-                                              float* __restrict__ part, Phys P, int S,
-                                              int nIblk, int ib0, int tile)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((PP & 1) ? 1 : 8, 8)))
+void k_pair32(const JRec32* __restrict__ jr, const JRec64* __restrict__ j64,
+              const Thr* __restrict__ thr, const double* __restrict__ st,
+              float* __restrict__ part, Phys P, int S, int nIblk, int ib0, int tile)
 {
     int s, ib;
     map_block(blockIdx.x, S, nIblk, s, ib);
@@ -843,7 +847,7 @@ __device__ __forceinline__ void flagged_pairs(const JRec64* __restrict__ jr,
     xp[(size_t)NACC * stride + idx] = ts;
 }
 
-__global__ __launch_bounds__(WG) void k_pair_fl(const JRec64* __restrict__ jr,
+inline __global__ __launch_bounds__(WG) void k_pair_fl(const JRec64* __restrict__ jr,
                                                const double* __restrict__ st,
                                                const int* __restrict__ flist,
                                                const int* __restrict__ fn,
@@ -942,7 +946,7 @@ __device__ __forceinline__ void hard_unit(const JRec64* __restrict__ jr,
 
 // The hard stars hlist[0 .. fn[1]): all in HSEG units each, or, beyond
 // HCAP of them, all whole; the workgroups take the units in turn.
-__global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr,
+inline __global__ __launch_bounds__(HWG) void k_hard_rows(const JRec64* __restrict__ jr,
                                                   const double* __restrict__ st,
                                                   const int* __restrict__ hlist,
                                                   const int* __restrict__ fn,
@@ -968,7 +972,7 @@ __device__ __forceinline__ void hard_sum(const double* __restrict__ hseg, int t,
     xh[(size_t)k * cap + star] = s;
 }
 
-__global__ __launch_bounds__(WG) void k_hard_sum(const int* __restrict__ hlist,
+inline __global__ __launch_bounds__(WG) void k_hard_sum(const int* __restrict__ hlist,
                                                 const int* __restrict__ fn,
                                                 const double* __restrict__ hseg,
                                                 double* __restrict__ xh, int cap)
@@ -1252,7 +1256,7 @@ __device__ __forceinline__ void erun64(const JRec64* __restrict__ jr, int j0, in
     }
 }
 
-__global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
+inline __global__ __launch_bounds__(WG) void k_epair64(const JRec64* __restrict__ jr,
                                                const double* __restrict__ st,
                                                double* __restrict__ part, Phys P, int S,
                                                int nIblk)
@@ -1317,7 +1321,7 @@ __device__ __forceinline__ void efin_star(const double* __restrict__ st,
                  + 0.5 * m0 * e * (G1 - mp * e));
 }
 
-__global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
+inline __global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
                                             const double* __restrict__ m,
                                             const double* __restrict__ part, int S,
                                             const double* __restrict__ gs,
@@ -1336,7 +1340,7 @@ __global__ __launch_bounds__(WG) void k_efin(const double* __restrict__ st,
 }
 
 // out = {kin, pot, pot_pn/c^2, pot_pn2/c^2, total}
-__global__ __launch_bounds__(WG) void k_ered(const double* __restrict__ eblk, int nblk,
+inline __global__ __launch_bounds__(WG) void k_ered(const double* __restrict__ eblk, int nblk,
                                             const double* __restrict__ st,
                                             const double* __restrict__ m,
                                             const double* __restrict__ gs,
